@@ -1,0 +1,656 @@
+// als.hip -- the ALS step of CP-ALS on gfx950 (src/cpd_gpu.cu:21-242 of the
+// reference does it with cuBLAS / cuSOLVER; the semantics kept here are the
+// CPU reference's, include/cpd.hpp:44-272).
+//
+//   gram       G = U^T U on MFMA: v_mfma_f64_16x16x4f64 for f64 factors,
+//              v_mfma_f32_16x16x4f32 for f32/bf16 factors.  One wave walks
+//              4-row steps of U; lane l holds U[r0 + l/16][16 t + l%16], which
+//              is both the A operand (A[i][k] = U[r0+k][16 ti + i]) and the B
+//              operand of tile (ti, tj).  Per-workgroup partial Grams are
+//              summed in a fixed order (deterministic).  Optionally the same
+//              pass scales U by 1/lambda first (normalisation) and forms the
+//              fit inner products sum_i U_ir M_ir.
+//   factorize  one workgroup: V = prod_{m != n} G_m (Hadamard), lower
+//              Cholesky in LAPACK dpotf2 order in LDS; when a pivot is not
+//              positive, cyclic Jacobi eigendecomposition of V and the
+//              pseudo-inverse V^+ = Q diag(1/w) Q^T with tolerance
+//              R * ulp(max |w|) (src/cpd_gpu.cu:487-489).
+//              On success the explicit inverse V^{-1} = L^{-T} L^{-1} is
+//              formed (so both outcomes leave one R x R matrix F).
+//   solve rows one thread per row: u = F m with F broadcast from LDS; the
+//              same pass forms per-workgroup column statistics for the norm
+//              (sum of squares, or max).
+//   lambda     one workgroup: lambda_r = sqrt(sum) (L2) or max(1, max) (max
+//              norm, src/kruskal_model.cpp:183-223), and 1/lambda (0 for a
+//              zero column).
+//   fit        one workgroup: fit = 1 - r / ||X|| with
+//              r^2 = ||X||^2 + |sum_ij l_i l_j prod_n G_n(i,j)| - 2 sum_r l_r ip_r
+//              (include/cpd.hpp:190-272).
+#include "als.hpp"
+#include "devcommon.hpp"
+
+namespace blco {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> __device__ __forceinline__ float as_f32(T x) { return (float)x; }
+template <> __device__ __forceinline__ float as_f32<bf16>(bf16 x) { return bf16_to_f32(x.bits); }
+template <typename T> __device__ __forceinline__ double as_f64(T x) { return (double)x; }
+template <> __device__ __forceinline__ double as_f64<bf16>(bf16 x) { return (double)bf16_to_f32(x.bits); }
+template <typename T> __device__ __forceinline__ bool is_zero(T x) { return x == (T)0; }
+
+// ------------------------------------------------------------------ gram
+
+// Partial Gram (and optional scaling / inner products) of rows
+// [g * rows_per_wg, ...) for workgroup g.  T tiles of 16 columns.
+template <typename UT, int T>
+__global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, int R,
+                                             uint64_t rows_per_wg,
+                                             const double* __restrict__ inv_scale,
+                                             const void* __restrict__ M,
+                                             double* __restrict__ part)
+{
+    constexpr bool F64 = std::is_same<UT, double>::value;
+    using AT = typename std::conditional<F64, double, float>::type;
+    using VT = typename std::conditional<F64, d4, f4>::type;
+    constexpr int NT = T * (T + 1) / 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    const uint64_t rb = (uint64_t)blockIdx.x * rows_per_wg;
+    const uint64_t re = min(rb + rows_per_wg, rows);
+    VT acc[NT];
+    #pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = VT{0, 0, 0, 0};
+    double ip[T];
+    #pragma unroll
+    for (int t = 0; t < T; ++t) ip[t] = 0.0;
+    double sc[T];
+    #pragma unroll
+    for (int t = 0; t < T; ++t) {
+        int col = t * 16 + i16;
+        sc[t] = (inv_scale && col < R) ? inv_scale[col] : 1.0;
+    }
+    for (uint64_t r0 = rb + (uint64_t)w * 4; r0 < re; r0 += WAVES_PER_WG * 4) {
+        const uint64_t row = r0 + k4;
+        AT a[T];
+        #pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int col = t * 16 + i16;
+            AT x = 0;
+            if (row < re && col < R) {
+                UT* p = U + row * (uint64_t)R + col;
+                if (inv_scale) {
+                    UT s = from_f64<UT>(as_f64(*p) * sc[t]);
+                    *p = s;
+                    x = (AT)as_f64(s);
+                } else {
+                    x = (AT)as_f64(*p);
+                }
+                if (M) {
+                    double mv = F64 ? reinterpret_cast<const double*>(M)[row * (uint64_t)R + col]
+                                    : (double)reinterpret_cast<const float*>(M)[row * (uint64_t)R + col];
+                    ip[t] += (double)x * mv;
+                }
+            }
+            a[t] = x;
+        }
+        int q = 0;
+        #pragma unroll
+        for (int ti = 0; ti < T; ++ti)
+            #pragma unroll
+            for (int tj = ti; tj < T; ++tj) {
+                if constexpr (F64)
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], a[tj], acc[q], 0, 0, 0);
+                else
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ti], a[tj], acc[q], 0, 0, 0);
+                ++q;
+            }
+    }
+    // combine the 4 waves in LDS, then write this workgroup's partial
+    __shared__ double s_g[WAVES_PER_WG][T * 16][T * 16 + 1];
+    __shared__ double s_ip[WAVES_PER_WG][T * 16];
+    int q = 0;
+    #pragma unroll
+    for (int ti = 0; ti < T; ++ti)
+        #pragma unroll
+        for (int tj = ti; tj < T; ++tj) {
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int i = ti * 16 + 4 * k4 + v, j = tj * 16 + i16;
+                double x = (double)acc[q][v];
+                s_g[w][i][j] = x;
+                s_g[w][j][i] = x;
+            }
+            ++q;
+        }
+    #pragma unroll
+    for (int t = 0; t < T; ++t) {
+        // lanes with equal i16 hold partial sums of the same column
+        double x = ip[t];
+        x += __shfl_xor(x, 16);
+        x += __shfl_xor(x, 32);
+        if (k4 == 0) s_ip[w][t * 16 + i16] = x;
+    }
+    __syncthreads();
+    double* out = part + (uint64_t)blockIdx.x * (uint64_t)(R * R + R);
+    for (int e = threadIdx.x; e < R * R; e += WG) {
+        int i = e / R, j = e % R;
+        out[e] = ((s_g[0][i][j] + s_g[1][i][j]) + s_g[2][i][j]) + s_g[3][i][j];
+    }
+    for (int e = threadIdx.x; e < R; e += WG)
+        out[R * R + e] = ((s_ip[0][e] + s_ip[1][e]) + s_ip[2][e]) + s_ip[3][e];
+}
+
+// Sums nparts partial blocks of (R*R + R) in order: G (R*R) and ip (R).
+__global__ __launch_bounds__(WG) void k_gram_reduce(const double* __restrict__ part, int nparts,
+                                                    int R, double* __restrict__ G,
+                                                    double* __restrict__ ip)
+{
+    const int stride = R * R + R;
+    for (int e = blockIdx.x * WG + threadIdx.x; e < stride; e += gridDim.x * WG) {
+        double s = 0.0;
+        for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * stride + e];
+        if (e < R * R) G[e] = s;
+        else if (ip) ip[e - R * R] = s;
+    }
+}
+
+// ------------------------------------------------------------- factorize
+
+// status: 0 Cholesky in fac (lower, row-major), 1 pseudo-inverse in fac.
+__global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mode, int R,
+                                                  double* __restrict__ fac,
+                                                  double* __restrict__ work,
+                                                  int* __restrict__ status)
+{
+    extern __shared__ double s_a[];  // R x R
+    __shared__ int s_fail;
+    __shared__ double s_piv;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < R * R; e += WG) {
+        double v = 1.0;
+        for (int m = 0; m < nmodes; ++m)
+            if (m != mode) v *= gs.g[m][e];
+        s_a[e] = v;
+        work[e] = v;  // keep V for the fallback
+    }
+    if (tid == 0) s_fail = 0;
+    __syncthreads();
+    // in-place lower Cholesky, dpotf2 order (column j: pivot, then below)
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            double ajj = s_a[j * R + j];
+            for (int k = 0; k < j; ++k) ajj -= s_a[j * R + k] * s_a[j * R + k];
+            if (!(ajj > 0.0)) s_fail = 1;
+            else s_piv = sqrt(ajj);
+            if (!s_fail) s_a[j * R + j] = s_piv;
+        }
+        __syncthreads();
+        if (s_fail) break;
+        const double piv = s_piv;
+        for (int i = j + 1 + tid; i < R; i += WG) {
+            double s = s_a[i * R + j];
+            for (int k = 0; k < j; ++k) s -= s_a[i * R + k] * s_a[j * R + k];
+            s_a[i * R + j] = s / piv;
+        }
+        __syncthreads();
+    }
+    if (!s_fail) {
+        // V^{-1} = L^{-T} L^{-1}: thread j forms column j of L^{-1} (forward
+        // substitution on e_j) into work, then every (a, b) entry is the dot
+        // of columns a and b of L^{-1} over k >= max(a, b).
+        double* Li = work;
+        for (int j = tid; j < R; j += WG) {
+            for (int i = 0; i < j; ++i) Li[i * R + j] = 0.0;
+            Li[j * R + j] = 1.0 / s_a[j * R + j];
+            for (int i = j + 1; i < R; ++i) {
+                double t = 0.0;
+                for (int k = j; k < i; ++k) t += s_a[i * R + k] * Li[k * R + j];
+                Li[i * R + j] = -t / s_a[i * R + i];
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int e = tid; e < R * R; e += WG) {
+            int a = e / R, b = e % R;
+            int k0 = a > b ? a : b;
+            double t = 0.0;
+            for (int k = k0; k < R; ++k) t += Li[k * R + a] * Li[k * R + b];
+            fac[e] = t;
+        }
+        if (tid == 0) *status = 0;
+        return;
+    }
+    // ---- fallback: symmetric pseudo-inverse by cyclic Jacobi
+    double* A = work;          // R x R, V (destroyed)
+    double* Q = work + R * R;  // R x R eigenvectors (columns)
+    double* wv = work + 2 * R * R;
+    for (int e = tid; e < R * R; e += WG) Q[e] = (e / R == e % R) ? 1.0 : 0.0;
+    __shared__ int s_done;
+    __shared__ double s_c, s_s;
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        if (tid == 0) {
+            double off = 0.0, diag = 0.0;
+            for (int i = 0; i < R; ++i)
+                for (int j = 0; j < R; ++j) {
+                    double x = A[i * R + j];
+                    if (i != j) off += x * x;
+                    else diag += x * x;
+                }
+            s_done = (off <= 1e-30 * diag || off == 0.0);
+        }
+        __syncthreads();
+        if (s_done) break;
+        for (int p = 0; p < R - 1; ++p)
+            for (int qq = p + 1; qq < R; ++qq) {
+                if (tid == 0) {
+                    double apq = A[p * R + qq];
+                    if (apq == 0.0) {
+                        s_c = 1.0; s_s = 0.0; s_done = 2;
+                    } else {
+                        double app = A[p * R + p], aqq = A[qq * R + qq];
+                        double tau = (aqq - app) / (2.0 * apq);
+                        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        double c = 1.0 / sqrt(1.0 + t * t);
+                        s_c = c; s_s = t * c; s_done = 0;
+                    }
+                }
+                __syncthreads();
+                if (s_done == 2) { __syncthreads(); continue; }
+                const double c = s_c, s = s_s;
+                for (int k = tid; k < R; k += WG) {
+                    double akp = A[k * R + p], akq = A[k * R + qq];
+                    A[k * R + p] = c * akp - s * akq;
+                    A[k * R + qq] = s * akp + c * akq;
+                }
+                __threadfence_block();
+                __syncthreads();
+                for (int k = tid; k < R; k += WG) {
+                    double apk = A[p * R + k], aqk = A[qq * R + k];
+                    A[p * R + k] = c * apk - s * aqk;
+                    A[qq * R + k] = s * apk + c * aqk;
+                    double qkp = Q[k * R + p], qkq = Q[k * R + qq];
+                    Q[k * R + p] = c * qkp - s * qkq;
+                    Q[k * R + qq] = s * qkp + c * qkq;
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
+    }
+    if (tid == 0) {
+        double emax = 0.0;
+        for (int i = 0; i < R; ++i) emax = fmax(emax, fabs(A[i * R + i]));
+        double tol = R * (nextafter(emax, emax + 1.0) - emax);
+        for (int i = 0; i < R; ++i) {
+            double x = A[i * R + i];
+            wv[i] = (fabs(x) > tol) ? 1.0 / x : 0.0;
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int e = tid; e < R * R; e += WG) {
+        int i = e / R, j = e % R;
+        double s = 0.0;
+        for (int k = 0; k < R; ++k) s += Q[i * R + k] * wv[k] * Q[j * R + k];
+        fac[e] = s;
+    }
+    if (tid == 0) *status = 1;
+}
+
+// ------------------------------------------------------------ solve rows
+
+// One thread per row: u = F m with F = V^{-1} (or V^+) from LDS, written as
+// the factor dtype; the same pass forms per-workgroup column statistics
+// (which 0: sum of squares, 1: max seeded with 1) into stats[blockIdx][R].
+template <typename MT, typename UT, int RR>
+__global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT* __restrict__ U,
+                                                   uint64_t rows, int R,
+                                                   const double* __restrict__ fac, int which,
+                                                   double* __restrict__ stats)
+{
+    extern __shared__ double s_f[];  // R x R, then WAVES_PER_WG x R stats
+    const int n = RR ? RR : R;
+    for (int e = threadIdx.x; e < n * n; e += WG) s_f[e] = fac[e];
+    double* s_st = s_f + n * n;
+    __syncthreads();
+    const uint64_t row = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    const bool valid = row < rows;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const MT* mrow = M + (valid ? row : 0) * (uint64_t)n;
+    double m[RR ? RR : 1];
+    if constexpr (RR > 0) {
+        #pragma unroll
+        for (int k = 0; k < RR; ++k) m[k] = valid ? (double)mrow[k] : 0.0;
+    }
+    #pragma unroll 1
+    for (int a = 0; a < n; ++a) {
+        const double* f = s_f + a * n;
+        double t = 0.0;
+        if constexpr (RR > 0) {
+            #pragma unroll
+            for (int k = 0; k < RR; ++k) t += f[k] * m[k];
+        } else {
+            for (int k = 0; k < n; ++k) t += f[k] * (valid ? (double)mrow[k] : 0.0);
+        }
+        if (valid) U[row * (uint64_t)n + a] = from_f64<UT>(t);
+        double x = (which == 0) ? (valid ? t * t : 0.0) : (valid ? fmax(t, 1.0) : 1.0);
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            double y = __shfl_xor(x, o);
+            x = (which == 0) ? x + y : fmax(x, y);
+        }
+        if (lane == 0) s_st[w * n + a] = x;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += WG) {
+        double x = s_st[i];
+        for (int ww = 1; ww < WAVES_PER_WG; ++ww)
+            x = (which == 0) ? x + s_st[ww * n + i] : fmax(x, s_st[ww * n + i]);
+        stats[(uint64_t)blockIdx.x * n + i] = x;
+    }
+}
+
+// lambda and 1/lambda from per-workgroup statistics.
+__global__ __launch_bounds__(WG) void k_lambda(const double* __restrict__ stats, int nparts, int R,
+                                               int which, double* __restrict__ lambda,
+                                               double* __restrict__ inv)
+{
+    for (int r = threadIdx.x; r < R; r += WG) {
+        double x = (which == 0) ? 0.0 : 1.0;
+        for (int p = 0; p < nparts; ++p) {
+            double y = stats[(uint64_t)p * R + r];
+            x = (which == 0) ? x + y : fmax(x, y);
+        }
+        if (which == 0) x = sqrt(x);
+        lambda[r] = x;
+        inv[r] = (x == 0.0) ? 0.0 : 1.0 / x;
+    }
+}
+
+// ------------------------------------------------------------------- fit
+
+__global__ __launch_bounds__(WG) void k_fit(GramSet gs, int nmodes, int R,
+                                            const double* __restrict__ lambda,
+                                            const double* __restrict__ ip, double normX2,
+                                            double* __restrict__ out)
+{
+    __shared__ double s_off[WG], s_diag[WG];
+    double off = 0.0, diag = 0.0;
+    for (int e = threadIdx.x; e < R * R; e += WG) {
+        int i = e / R, j = e % R;
+        if (j > i) continue;
+        double t = lambda[i] * lambda[j];
+        for (int m = 0; m < nmodes; ++m) t *= gs.g[m][e];
+        if (j < i) off += 2.0 * t;
+        else diag += t;
+    }
+    s_off[threadIdx.x] = off;
+    s_diag[threadIdx.x] = diag;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double so = 0.0, sd = 0.0, inner = 0.0;
+        for (int t = 0; t < WG; ++t) { so += s_off[t]; sd += s_diag[t]; }
+        for (int r = 0; r < R; ++r) inner += ip[r] * lambda[r];
+        double normU = fabs(so + sd);
+        double res = normX2 + normU - 2.0 * inner;
+        if (res > 0.0) res = sqrt(res);
+        out[0] = 1.0 - res / sqrt(normX2);
+        out[1] = inner;
+        out[2] = normU;
+    }
+}
+
+// Sum of squares of n values (f64 or f32 storage) per workgroup.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_sumsq(const T* __restrict__ x, uint64_t n,
+                                              double* __restrict__ part)
+{
+    __shared__ double s[WAVES_PER_WG];
+    double acc = 0.0;
+    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < n; e += (uint64_t)gridDim.x * WG) {
+        double v = (double)x[e];
+        acc += v * v;
+    }
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += part[i];
+        out[0] = s;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_hadamard(double* __restrict__ x, const double* __restrict__ y,
+                                                 uint64_t n)
+{
+    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < n; e += (uint64_t)gridDim.x * WG)
+        x[e] *= y[e];
+}
+
+// Per-workgroup column statistics of an existing factor (which 0: sum of
+// squares, 1: max seeded with 1), for the stand-alone column norm.
+template <typename UT>
+__global__ __launch_bounds__(WG) void k_colstats(const UT* __restrict__ U, uint64_t rows, int R,
+                                                 uint64_t rows_per_wg, int which,
+                                                 double* __restrict__ stats)
+{
+    const uint64_t rb = (uint64_t)blockIdx.x * rows_per_wg;
+    const uint64_t re = min(rb + rows_per_wg, rows);
+    for (int c = threadIdx.x; c < R; c += WG) {
+        double x = (which == 0) ? 0.0 : 1.0;
+        for (uint64_t i = rb; i < re; ++i) {
+            double v = as_f64(U[i * (uint64_t)R + c]);
+            x = (which == 0) ? x + v * v : fmax(x, v);
+        }
+        stats[(uint64_t)blockIdx.x * R + c] = x;
+    }
+}
+
+template <typename UT>
+__global__ __launch_bounds__(WG) void k_scale(UT* __restrict__ U, uint64_t cells, int R,
+                                              const double* __restrict__ inv)
+{
+    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < cells; e += (uint64_t)gridDim.x * WG)
+        U[e] = from_f64<UT>(as_f64(U[e]) * inv[e % R]);
+}
+
+__global__ void k_vecsum(double* __restrict__ x, uint64_t n)
+{
+    __shared__ double s[WG];
+    double acc = 0.0;
+    for (uint64_t e = threadIdx.x; e < n; e += WG) acc += x[e];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < WG; ++i) t += s[i];
+        x[0] = t;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+template <typename UT>
+static int gram_t(UT* U, uint64_t rows, int R, const double* inv, const void* M, double* part,
+                  int nparts, uint64_t rpw, hipStream_t st)
+{
+    int T = (R + 15) / 16;
+    switch (T) {
+        case 1: hipLaunchKernelGGL((k_gram<UT, 1>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
+        case 2: hipLaunchKernelGGL((k_gram<UT, 2>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
+        case 3: hipLaunchKernelGGL((k_gram<UT, 3>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
+        case 4: hipLaunchKernelGGL((k_gram<UT, 4>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
+        default: return fail(BLCO_EUNSUPPORTED, "rank %d > 64 not supported by the Gram kernel", R);
+    }
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg)
+{
+    uint64_t steps = (rows + 3) / 4;
+    uint64_t want = (steps + 15) / 16;  // ~16 row-steps per workgroup (4 per wave)
+    if (want < 1) want = 1;
+    if (want > 512) want = 512;  // callers size partial buffers for 512 parts
+    uint64_t rpw = (rows + want - 1) / want;
+    rpw = (rpw + 3) / 4 * 4;
+    if (rpw == 0) rpw = 4;
+    *rows_per_wg = rpw;
+    *nparts = (int)((rows + rpw - 1) / rpw);
+    if (*nparts < 1) *nparts = 1;
+    return BLCO_OK;
+}
+
+int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
+                double* part, double* G, double* ip, hipStream_t st)
+{
+    if (R < 1 || R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] for the Gram kernel", R);
+    int nparts;
+    uint64_t rpw;
+    gram_parts(rows, &nparts, &rpw);
+    int rc;
+    if (dtype == BLCO_F64) rc = gram_t<double>((double*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
+    else if (dtype == BLCO_F32) rc = gram_t<float>((float*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
+    else rc = gram_t<bf16>((bf16*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
+    if (rc) return rc;
+    int stride = R * R + R;
+    hipLaunchKernelGGL(k_gram_reduce, dim3((stride + WG - 1) / WG), dim3(WG), 0, st, part, nparts, R, G, ip);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double* fac, double* work,
+                     int* status, hipStream_t st)
+{
+    size_t lds = sizeof(double) * (size_t)R * R;
+    if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute((const void*)k_factorize,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_factorize, dim3(1), dim3(WG), lds, st, gs, nmodes, mode, R, fac, work, status);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+template <typename MT, typename UT>
+static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac,
+                        int which, double* stats, hipStream_t st)
+{
+    uint32_t grid = (uint32_t)((rows + WG - 1) / WG);
+    if (grid == 0) return BLCO_OK;
+    size_t lds = sizeof(double) * ((size_t)R * R + (size_t)WAVES_PER_WG * R);
+    const void* k;
+    switch (R) {
+        case 8: k = (const void*)&k_solve_rows<MT, UT, 8>; break;
+        case 16: k = (const void*)&k_solve_rows<MT, UT, 16>; break;
+        case 32: k = (const void*)&k_solve_rows<MT, UT, 32>; break;
+        case 64: k = (const void*)&k_solve_rows<MT, UT, 64>; break;
+        default:
+            if (R > 128) return fail(BLCO_EUNSUPPORTED, "rank %d > 128 in the row solve", R);
+            k = (const void*)&k_solve_rows<MT, UT, 0>;
+            break;
+    }
+    if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* params[] = {(void*)&M, (void*)&U, (void*)&rows, (void*)&R, (void*)&fac,
+                      (void*)&which, (void*)&stats};
+    HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(WG), params, lds, st));
+    return BLCO_OK;
+}
+
+int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
+                      int which, double* stats, hipStream_t st)
+{
+    if (dtype == BLCO_F64) return solve_rows_t<double, double>((const double*)M, (double*)U, rows, R, fac, which, stats, st);
+    if (dtype == BLCO_F32) return solve_rows_t<float, float>((const float*)M, (float*)U, rows, R, fac, which, stats, st);
+    return solve_rows_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, which, stats, st);
+}
+
+uint32_t solve_parts(uint64_t rows) { return (uint32_t)((rows + WG - 1) / WG); }
+
+int launch_lambda(const double* stats, int nparts, int R, int which, double* lambda, double* inv,
+                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(WG), 0, st, stats, nparts, R, which, lambda, inv);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_fit(const GramSet& gs, int nmodes, int R, const double* lambda, const double* ip,
+               double normX2, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(WG), 0, st, gs, nmodes, R, lambda, ip, normX2, out);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_sumsq(const void* x, int dtype, uint64_t n, double* part, int nparts, double* out,
+                 hipStream_t st)
+{
+    if (dtype == BLCO_F64)
+        hipLaunchKernelGGL(k_sumsq<double>, dim3(nparts), dim3(WG), 0, st, (const double*)x, n, part);
+    else
+        hipLaunchKernelGGL(k_sumsq<float>, dim3(nparts), dim3(WG), 0, st, (const float*)x, n, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part, nparts, out);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_hadamard(double* x, const double* y, uint64_t n, hipStream_t st)
+{
+    uint32_t grid = (uint32_t)std::min<uint64_t>((n + WG - 1) / WG, 4096);
+    if (grid == 0) return BLCO_OK;
+    hipLaunchKernelGGL(k_hadamard, dim3(grid), dim3(WG), 0, st, x, y, n);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+}  // namespace blco
+
+namespace blco {
+
+template <typename UT>
+static void colstats_t(const UT* U, uint64_t rows, int R, uint64_t rpw, int nparts, int which,
+                       double* stats, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_colstats<UT>, dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, which, stats);
+}
+
+int launch_colstats(const void* U, int dtype, uint64_t rows, int R, int which, double* stats,
+                    int nparts, hipStream_t st)
+{
+    uint64_t rpw = (rows + nparts - 1) / nparts;
+    if (rpw == 0) rpw = 1;
+    if (dtype == BLCO_F64) colstats_t((const double*)U, rows, R, rpw, nparts, which, stats, st);
+    else if (dtype == BLCO_F32) colstats_t((const float*)U, rows, R, rpw, nparts, which, stats, st);
+    else colstats_t((const bf16*)U, rows, R, rpw, nparts, which, stats, st);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_scale(void* U, int dtype, uint64_t rows, int R, const double* inv, hipStream_t st)
+{
+    uint64_t cells = rows * (uint64_t)R;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((cells + WG - 1) / WG, 8192);
+    if (grid == 0) return BLCO_OK;
+    if (dtype == BLCO_F64) hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(WG), 0, st, (double*)U, cells, R, inv);
+    else if (dtype == BLCO_F32) hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(WG), 0, st, (float*)U, cells, R, inv);
+    else hipLaunchKernelGGL(k_scale<bf16>, dim3(grid), dim3(WG), 0, st, (bf16*)U, cells, R, inv);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_vecsum(double* x, uint64_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vecsum, dim3(1), dim3(WG), 0, st, x, n);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+}  // namespace blco

@@ -1,0 +1,43 @@
+// als.hpp -- launchers of als.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "blco_host.h"
+
+namespace blco {
+
+struct GramSet {
+    const double* g[BLCO_MAX_MODES];
+};
+
+int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg);
+// G = U^T U (f64, R x R) from dtype U; with inv_scale, U is first scaled in
+// place by inv_scale[col]; with M, ip[col] = sum_i U_ic M_ic.  part needs
+// gram_parts(rows) * (R*R + R) doubles.
+int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
+                double* part, double* G, double* ip, hipStream_t st);
+// fac: R x R (V^{-1}, or V^+ with status 1); work: 2 R^2 + R doubles.
+int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double* fac, double* work,
+                     int* status, hipStream_t st);
+uint32_t solve_parts(uint64_t rows);
+// stats: solve_parts(rows) * R doubles.
+int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
+                      int which, double* stats, hipStream_t st);
+int launch_lambda(const double* stats, int nparts, int R, int which, double* lambda, double* inv,
+                  hipStream_t st);
+// out[0] = fit, out[1] = inner product, out[2] = ||M||^2.
+int launch_fit(const GramSet& gs, int nmodes, int R, const double* lambda, const double* ip,
+               double normX2, double* out, hipStream_t st);
+int launch_sumsq(const void* x, int dtype, uint64_t n, double* part, int nparts, double* out,
+                 hipStream_t st);
+int launch_hadamard(double* x, const double* y, uint64_t n, hipStream_t st);
+
+int launch_colstats(const void* U, int dtype, uint64_t rows, int R, int which, double* stats,
+                    int nparts, hipStream_t st);
+int launch_scale(void* U, int dtype, uint64_t rows, int R, const double* inv, hipStream_t st);
+int launch_vecsum(double* x, uint64_t n, hipStream_t st);
+
+}  // namespace blco

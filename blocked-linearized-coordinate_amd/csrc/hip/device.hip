@@ -1,0 +1,550 @@
+// device.hip -- BLCO tensors in HBM (or streamed from pinned host memory),
+// MTTKRP dispatch and the stand-alone ALS-step entry points of the C ABI.
+//
+// Layout in HBM: coords u64[nnz] and values (f64 or f32)[nnz] in ALTO order,
+// one contiguous allocation each, blocks back to back (block b occupies
+// [block_start[b], block_start[b] + block_nnz[b])); block coordinates
+// u64[nblocks][nmodes]; a chunk table per tiles-per-wave setting (one entry
+// per wave of work: begin, length, block).  Streamed tensors keep coords and
+// values in pinned host memory and pass them through two device slots of
+// slot_nnz nonzeros, copying on a side stream while the previous slot is
+// being consumed (src/alto_dev.cu:1869-1874, 1987 stream one block per
+// launch through NUM_STREAMS slots).
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "als.hpp"
+#include "device.hpp"
+#include "devcommon.hpp"
+#include "mttkrp.hpp"
+
+using namespace blco;
+
+namespace blco {
+
+// LDS for a privatised output: with the <= 12 KiB of per-wave staging the
+// workgroup stays within the 64 KiB a launch gets without an attribute.
+constexpr uint64_t LDS_OUT_MAX = 48 * 1024;
+constexpr int LVL3_MAX_MODE_LENGTH = 100;     // src/utils.hpp:13
+constexpr int DEFAULT_TPW = 4;                // tiles of 64 per wave
+constexpr int DEFAULT_NPRTN = 1;
+constexpr uint32_t LVL3_CHUNKS_PER_WG = 64;
+
+static int value_size(int vdt) { return vdt == BLCO_F64 ? 8 : 4; }
+
+const ChunkTab* chunk_table(blco_dev* d, int tpw)
+{
+    for (auto& t : d->tabs)
+        if (t.tpw == tpw) return &t;
+    ChunkTab t;
+    t.tpw = tpw;
+    const uint64_t CH = (uint64_t)WAVE * tpw;
+    std::vector<Chunk> h;
+    t.first.push_back(0);
+    for (uint64_t b = 0; b < d->nblocks; ++b) {
+        for (uint64_t s = 0; s < d->bnnz[b]; s += CH) {
+            Chunk c;
+            c.begin = d->bstart[b] + s;
+            c.len = (uint32_t)std::min<uint64_t>(CH, d->bnnz[b] - s);
+            c.block = (uint32_t)b;
+            h.push_back(c);
+        }
+        t.first.push_back((uint32_t)h.size());
+    }
+    t.n = (uint32_t)h.size();
+    // streaming units: chunk ranges of one block holding at most slot_nnz
+    if (d->stream_data) {
+        for (uint64_t b = 0; b < d->nblocks; ++b) {
+            uint32_t c0 = t.first[b], c1 = t.first[b + 1];
+            uint32_t u0 = c0;
+            uint64_t acc = 0;
+            for (uint32_t c = c0; c < c1; ++c) {
+                if (acc + h[c].len > d->slot_nnz && c > u0) {
+                    t.units.push_back({u0, c, h[u0].begin, acc});
+                    u0 = c;
+                    acc = 0;
+                }
+                acc += h[c].len;
+            }
+            if (c1 > u0) t.units.push_back({u0, c1, h[u0].begin, acc});
+        }
+    }
+    if (t.n) {
+        if (hipMalloc(&t.d, sizeof(Chunk) * t.n) != hipSuccess) {
+            fail(BLCO_ENOMEM, "device allocation of the chunk table failed");
+            return nullptr;
+        }
+        if (hipMemcpy(t.d, h.data(), sizeof(Chunk) * t.n, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(t.d);
+            fail(BLCO_EHIP, "upload of the chunk table failed");
+            return nullptr;
+        }
+        d->bytes += sizeof(Chunk) * t.n;
+    }
+    d->tabs.push_back(t);
+    return &d->tabs.back();
+}
+
+static bool lvl3_fits(const blco_dev* d, int mode, int rank, int ftype)
+{
+    uint64_t esz = ftype == BLCO_F64 ? 8 : 4;
+    return d->dims[mode] * (uint64_t)rank * esz <= LDS_OUT_MAX;
+}
+
+int auto_kernel(const blco_dev* d, int mode, int rank, int ftype)
+{
+    // src/alto_dev.cu:1975,1996-2001: short modes -> lvl3, else lvl1.
+    if (d->dims[mode] <= (uint64_t)LVL3_MAX_MODE_LENGTH && lvl3_fits(d, mode, rank, ftype))
+        return BLCO_K_LVL3;
+    return BLCO_K_BATCHED;
+}
+
+static int ensure_partials(blco_dev* d, size_t bytes)
+{
+    if (d->partials_bytes >= bytes) return BLCO_OK;
+    if (d->partials) (void)hipFree(d->partials);
+    d->partials = nullptr;
+    d->partials_bytes = 0;
+    HIP_TRY(hipMalloc(&d->partials, bytes));
+    d->partials_bytes = bytes;
+    return BLCO_OK;
+}
+
+// One kernel launch over chunks [c0, c1) whose data lives at coords/vals
+// indexed from coord_base.
+static int launch_range(blco_dev* d, int which, MttkrpArgs a, const ChunkTab* t, uint32_t c0,
+                        uint32_t c1, int ftype, hipStream_t st)
+{
+    if (c1 <= c0) return BLCO_OK;
+    a.chunks = t->d;
+    a.chunk0 = c0;
+    a.nchunks = c1 - c0;
+    uint32_t grid;
+    size_t lds = 0;
+    if (which == W_LVL3) {
+        a.chunks_per_wg = LVL3_CHUNKS_PER_WG;
+        grid = (a.nchunks + LVL3_CHUNKS_PER_WG - 1) / LVL3_CHUNKS_PER_WG;
+        lds = d->dims[a.mode] * (uint64_t)a.rank * (ftype == BLCO_F64 ? 8 : 4);
+    } else {
+        grid = (a.nchunks + WAVES_PER_WG - 1) / WAVES_PER_WG;
+    }
+    return launch_mttkrp(which, a, ftype, grid, lds, st);
+}
+
+int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
+           int nprtn, int tpw, hipStream_t st, int* used_kernel)
+{
+    if (!d) return fail(BLCO_EINVAL, "null tensor");
+    if (mode < 0 || mode >= d->nmodes) return fail(BLCO_EINVAL, "mode %d outside [0, %d)", mode, d->nmodes);
+    if (rank < 1 || rank > 4096) return fail(BLCO_EINVAL, "rank %d outside [1, 4096]", rank);
+    if (ftype != BLCO_F64 && ftype != BLCO_F32 && ftype != BLCO_BF16)
+        return fail(BLCO_EINVAL, "factor dtype %d", ftype);
+    const int need_v = ftype == BLCO_F64 ? BLCO_F64 : BLCO_F32;
+    if (d->vdtype != need_v)
+        return fail(BLCO_EUNSUPPORTED, "%s factors need %s values on the device",
+                    ftype == BLCO_F64 ? "f64" : "f32/bf16", need_v == BLCO_F64 ? "f64" : "f32");
+    if (!factors || !out) return fail(BLCO_EINVAL, "null factors or output");
+    for (int m = 0; m < d->nmodes; ++m)
+        if (m != mode && !factors[m]) return fail(BLCO_EINVAL, "factor %d is null", m);
+    if (d->nmodes < 2) return fail(BLCO_EUNSUPPORTED, "MTTKRP needs at least 2 modes");
+    if (tpw <= 0) tpw = DEFAULT_TPW;
+    if (tpw > 64) return fail(BLCO_EINVAL, "thread_cf %d > 64", tpw);
+    if (nprtn <= 0) nprtn = DEFAULT_NPRTN;
+    if (nprtn > 1024) return fail(BLCO_EINVAL, "nprtn %d > 1024", nprtn);
+    if (!st) st = d->stream;
+    if (kernel == BLCO_K_AUTO || kernel == BLCO_K_SELECT) kernel = auto_kernel(d, mode, rank, ftype);
+    int which;
+    switch (kernel) {
+        case BLCO_K_LVL1:
+        case BLCO_K_LVL2:
+        case BLCO_K_BATCHED:
+        case BLCO_K_BATCHED2: which = W_LVL1; break;
+        case BLCO_K_LVL3:
+            which = W_LVL3;
+            if (!lvl3_fits(d, mode, rank, ftype)) {
+                // the reference warns and proceeds (src/alto_dev.cu:1866); a
+                // privatised copy must fit in LDS here, so lvl1 runs instead
+                which = W_LVL1;
+                kernel = BLCO_K_BATCHED;
+            }
+            break;
+        case BLCO_K_ATOMIC: which = W_ATOMIC; break;
+        default: return fail(BLCO_EINVAL, "unknown kernel id %d", kernel);
+    }
+    if (used_kernel) *used_kernel = kernel;
+    if (d->nnz == 0) return BLCO_OK;
+    const ChunkTab* t = chunk_table(d, tpw);
+    if (!t) return BLCO_ENOMEM;
+
+    MttkrpArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.bcoords = d->d_bcoords;
+    for (int m = 0; m < d->nmodes; ++m) {
+        a.mask[m] = d->masks[m];
+        a.pos[m] = d->pos[m];
+        a.f[m] = factors[m];
+    }
+    a.nmodes = d->nmodes;
+    a.mode = mode;
+    a.rank = rank;
+    a.out = out;
+    a.out_rows = d->dims[mode];
+    a.nprtn = which == W_LVL3 ? nprtn : 1;
+    const int otype = ftype == BLCO_F64 ? BLCO_F64 : BLCO_F32;
+    const uint64_t cells = d->dims[mode] * (uint64_t)rank;
+    if (which == W_LVL3 && nprtn > 1) {
+        size_t bytes = (size_t)nprtn * cells * (otype == BLCO_F64 ? 8 : 4);
+        int rc = ensure_partials(d, bytes);
+        if (rc) return rc;
+        a.partials = d->partials;
+        rc = launch_fill(d->partials, (uint64_t)nprtn * cells, otype, 0.0, st);
+        if (rc) return rc;
+    }
+
+    int rc = BLCO_OK;
+    if (!d->stream_data) {
+        a.coords = d->d_coords;
+        a.vals = d->d_vals;
+        a.coord_base = 0;
+        if (kernel == BLCO_K_LVL1 || kernel == BLCO_K_LVL2) {
+            for (uint64_t b = 0; b < d->nblocks && !rc; ++b)
+                rc = launch_range(d, which, a, t, t->first[b], t->first[b + 1], ftype, st);
+        } else {
+            rc = launch_range(d, which, a, t, 0, t->n, ftype, st);
+        }
+    } else {
+        // double-buffered streaming: unit u goes through slot u % 2
+        const int vsz = value_size(d->vdtype);
+        for (size_t u = 0; u < t->units.size() && !rc; ++u) {
+            const StreamUnit& su = t->units[u];
+            const int s = (int)(u & 1);
+            HIP_TRY(hipStreamWaitEvent(d->copy_stream, d->ev_done[s], 0));
+            HIP_TRY(hipMemcpyAsync(d->slot_coords[s], d->h_coords + su.begin, su.nnz * 8,
+                                   hipMemcpyHostToDevice, d->copy_stream));
+            HIP_TRY(hipMemcpyAsync(d->slot_vals[s], (const char*)d->h_vals + su.begin * vsz,
+                                   su.nnz * vsz, hipMemcpyHostToDevice, d->copy_stream));
+            HIP_TRY(hipEventRecord(d->ev_copied[s], d->copy_stream));
+            HIP_TRY(hipStreamWaitEvent(st, d->ev_copied[s], 0));
+            a.coords = d->slot_coords[s];
+            a.vals = d->slot_vals[s];
+            a.coord_base = su.begin;
+            rc = launch_range(d, which, a, t, su.c0, su.c1, ftype, st);
+            HIP_TRY(hipEventRecord(d->ev_done[s], st));
+        }
+    }
+    if (rc) return rc;
+    if (which == W_LVL3 && nprtn > 1)
+        rc = launch_reduce_partials(out, d->partials, cells, nprtn, otype, st);
+    return rc;
+}
+
+static bool have_kfd() { return access("/dev/kfd", F_OK) == 0; }
+
+}  // namespace blco
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+int blco_device_count(void)
+{
+    if (!have_kfd()) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos,
+                           uint64_t nnz, const uint64_t* coords, const double* vals,
+                           uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz,
+                           const uint64_t* block_coords, int value_dtype, int stream_data,
+                           uint64_t slot_nnz, blco_dev** out)
+{
+    if (!out) return fail(BLCO_EINVAL, "null output");
+    *out = nullptr;
+    if (nmodes < 1 || nmodes > BLCO_MAX_MODES) return fail(BLCO_EINVAL, "nmodes %d", nmodes);
+    if (!dims || !masks || !pos) return fail(BLCO_EINVAL, "null layout");
+    if (nnz && (!coords || !vals || !block_start || !block_nnz || !block_coords || nblocks == 0))
+        return fail(BLCO_EINVAL, "null tensor arrays");
+    if (value_dtype != BLCO_F64 && value_dtype != BLCO_F32)
+        return fail(BLCO_EINVAL, "value dtype %d", value_dtype);
+    uint64_t sum = 0;
+    for (uint64_t b = 0; b < nblocks; ++b) {
+        if (block_start[b] != sum) return fail(BLCO_EINVAL, "blocks are not contiguous at %llu", (unsigned long long)b);
+        sum += block_nnz[b];
+    }
+    if (sum != nnz) return fail(BLCO_EINVAL, "block sizes sum to %llu, nnz is %llu", (unsigned long long)sum, (unsigned long long)nnz);
+    for (int m = 0; m < nmodes; ++m) {
+        if (dims[m] == 0) return fail(BLCO_EINVAL, "mode %d has length 0", m);
+        if (masks[m] && (pos[m] < 0 || pos[m] > 63)) return fail(BLCO_EINVAL, "mode %d position %d", m, pos[m]);
+    }
+    // every decoded index must lie inside its mode: kernels gather factor
+    // rows and scatter output rows with these indices unchecked
+    for (uint64_t b = 0; b < nblocks; ++b) {
+        for (int m = 0; m < nmodes; ++m) {
+            const uint64_t bc = block_coords[b * nmodes + m], mk = masks[m];
+            const int ps = mk ? pos[m] : 0;
+            uint64_t mx = 0;
+            for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
+                uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                mx = v > mx ? v : mx;
+            }
+            if (block_nnz[b] && mx >= dims[m])
+                return fail(BLCO_EINVAL, "block %llu decodes mode-%d index %llu >= %llu",
+                            (unsigned long long)b, m, (unsigned long long)mx,
+                            (unsigned long long)dims[m]);
+        }
+    }
+    if (!have_kfd()) return fail(BLCO_ENODEV, "no GPU (/dev/kfd absent)");
+    if (stream_data && slot_nnz == 0) slot_nnz = 1ull << 24;  // --max-block-size default
+    // a slot holds at least one chunk at the largest tiles-per-wave (64 x 64)
+    if (stream_data && slot_nnz < 4096) slot_nnz = 4096;
+
+    blco_dev* d = new (std::nothrow) blco_dev();
+    if (!d) return fail(BLCO_ENOMEM, "out of host memory");
+    d->nmodes = nmodes;
+    for (int m = 0; m < nmodes; ++m) {
+        d->dims[m] = dims[m];
+        d->masks[m] = masks[m];
+        d->pos[m] = masks[m] ? pos[m] : 0;  // a 0-bit mode never shifts
+    }
+    d->nnz = nnz;
+    d->vdtype = value_dtype;
+    d->stream_data = stream_data ? 1 : 0;
+    d->slot_nnz = slot_nnz;
+    d->nblocks = nblocks;
+    d->bstart.assign(block_start, block_start + nblocks);
+    d->bnnz.assign(block_nnz, block_nnz + nblocks);
+    auto cleanup = [&](int rc) { blco_dev_free(d); return rc; };
+    if (hipGetDevice(&d->device) != hipSuccess) return cleanup(fail(BLCO_EHIP, "hipGetDevice failed"));
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(BLCO_EHIP, "stream creation failed"));
+    if (nblocks) {
+        if (hipMalloc(&d->d_bcoords, sizeof(uint64_t) * nblocks * nmodes) != hipSuccess)
+            return cleanup(fail(BLCO_ENOMEM, "device allocation failed (block coordinates)"));
+        if (hipMemcpy(d->d_bcoords, block_coords, sizeof(uint64_t) * nblocks * nmodes,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(BLCO_EHIP, "upload of block coordinates failed"));
+        d->bytes += sizeof(uint64_t) * nblocks * nmodes;
+    }
+    const int vsz = value_size(value_dtype);
+    // values converted to the device dtype through a bounded host staging buffer
+    auto convert = [&](uint64_t off, uint64_t cnt, void* dst) {
+        if (value_dtype == BLCO_F64) std::memcpy(dst, vals + off, cnt * 8);
+        else {
+            float* f = (float*)dst;
+            for (uint64_t i = 0; i < cnt; ++i) f[i] = (float)vals[off + i];
+        }
+    };
+    if (!d->stream_data) {
+        if (nnz) {
+            if (hipMalloc(&d->d_coords, nnz * 8) != hipSuccess || hipMalloc(&d->d_vals, nnz * vsz) != hipSuccess)
+                return cleanup(fail(BLCO_ENOMEM, "device allocation of %llu nonzeros failed",
+                                    (unsigned long long)nnz));
+            d->bytes += nnz * (8 + vsz);
+            if (hipMemcpy(d->d_coords, coords, nnz * 8, hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(fail(BLCO_EHIP, "coordinate upload failed"));
+            const uint64_t STAGE = 1ull << 24;
+            std::vector<char> buf;
+            try { buf.resize(std::min(nnz, STAGE) * vsz); } catch (...) { return cleanup(fail(BLCO_ENOMEM, "staging")); }
+            for (uint64_t off = 0; off < nnz; off += STAGE) {
+                uint64_t cnt = std::min(STAGE, nnz - off);
+                convert(off, cnt, buf.data());
+                if (hipMemcpy((char*)d->d_vals + off * vsz, buf.data(), cnt * vsz, hipMemcpyHostToDevice) != hipSuccess)
+                    return cleanup(fail(BLCO_EHIP, "value upload failed"));
+            }
+        }
+    } else {
+        if (nnz) {
+            if (hipHostMalloc((void**)&d->h_coords, nnz * 8, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc(&d->h_vals, nnz * vsz, hipHostMallocDefault) != hipSuccess)
+                return cleanup(fail(BLCO_ENOMEM, "pinned host allocation of %llu nonzeros failed",
+                                    (unsigned long long)nnz));
+            std::memcpy(d->h_coords, coords, nnz * 8);
+            convert(0, nnz, d->h_vals);
+        }
+        uint64_t cap = std::min<uint64_t>(slot_nnz, nnz ? nnz : 1);
+        for (int s = 0; s < 2; ++s) {
+            if (hipMalloc(&d->slot_coords[s], cap * 8) != hipSuccess || hipMalloc(&d->slot_vals[s], cap * vsz) != hipSuccess)
+                return cleanup(fail(BLCO_ENOMEM, "device slot allocation failed"));
+            d->bytes += cap * (8 + vsz);
+            if (hipEventCreateWithFlags(&d->ev_copied[s], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&d->ev_done[s], hipEventDisableTiming) != hipSuccess)
+                return cleanup(fail(BLCO_EHIP, "event creation failed"));
+        }
+        if (hipStreamCreateWithFlags(&d->copy_stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(BLCO_EHIP, "copy stream creation failed"));
+    }
+    // every block must fit a slot when streaming (chunks never straddle blocks)
+    *out = d;
+    return BLCO_OK;
+}
+
+int blco_dev_create(const blco_host* h, int value_dtype, int stream_data, uint64_t slot_nnz,
+                    blco_dev** out)
+{
+    if (!h) return fail(BLCO_EINVAL, "null host tensor");
+    return blco_dev_create_arrays(h->L.nmodes, h->L.dims, h->L.masks, h->L.pos, h->nnz,
+                                  h->coords.data(), h->vals.data(), h->block_nnz.size(),
+                                  h->block_start.data(), h->block_nnz.data(),
+                                  h->block_coords.data(), value_dtype, stream_data, slot_nnz, out);
+}
+
+void blco_dev_free(blco_dev* d)
+{
+    if (!d) return;
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    if (d->copy_stream) (void)hipStreamSynchronize(d->copy_stream);
+    for (auto& t : d->tabs)
+        if (t.d) (void)hipFree(t.d);
+    if (d->d_bcoords) (void)hipFree(d->d_bcoords);
+    if (d->d_coords) (void)hipFree(d->d_coords);
+    if (d->d_vals) (void)hipFree(d->d_vals);
+    if (d->partials) (void)hipFree(d->partials);
+    for (int s = 0; s < 2; ++s) {
+        if (d->slot_coords[s]) (void)hipFree(d->slot_coords[s]);
+        if (d->slot_vals[s]) (void)hipFree(d->slot_vals[s]);
+        if (d->ev_copied[s]) (void)hipEventDestroy(d->ev_copied[s]);
+        if (d->ev_done[s]) (void)hipEventDestroy(d->ev_done[s]);
+    }
+    if (d->h_coords) (void)hipHostFree(d->h_coords);
+    if (d->h_vals) (void)hipHostFree(d->h_vals);
+    if (d->copy_stream) (void)hipStreamDestroy(d->copy_stream);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+uint64_t blco_dev_bytes(const blco_dev* d) { return d ? d->bytes : 0; }
+
+int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype, void* const* factors, void* out,
+                int kernel, int nprtn, int thread_cf, void* stream)
+{
+    return mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, thread_cf,
+                  (hipStream_t)stream, nullptr);
+}
+
+int blco_auto_kernel(const blco_dev* d, int mode, int rank)
+{
+    if (!d || mode < 0 || mode >= d->nmodes) return fail(BLCO_EINVAL, "bad mode");
+    return auto_kernel(d, mode, rank, d->vdtype);
+}
+
+// ----------------------------------------------------- ALS-step primitives
+
+int blco_gram(const void* U, int dtype, uint64_t rows, int rank, double* G, void* stream)
+{
+    if (!U || !G || rank < 1 || rank > 64) return fail(BLCO_EINVAL, "bad Gram arguments");
+    hipStream_t st = (hipStream_t)stream;
+    int nparts;
+    uint64_t rpw;
+    gram_parts(rows, &nparts, &rpw);
+    double* part = nullptr;
+    HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)nparts * (rank * rank + rank)));
+    int rc = launch_gram(const_cast<void*>(U), dtype, rows, rank, nullptr, nullptr, part, G, nullptr, st);
+    hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(part);
+    if (rc) return rc;
+    HIP_TRY(e);
+    return BLCO_OK;
+}
+
+int blco_hadamard(double* x, const double* y, uint64_t n, void* stream)
+{
+    if (!x || !y) return fail(BLCO_EINVAL, "null vector");
+    return launch_hadamard(x, y, n, (hipStream_t)stream);
+}
+
+int blco_value_fill(double* x, uint64_t n, double val, void* stream)
+{
+    if (!x) return fail(BLCO_EINVAL, "null vector");
+    return launch_fill(x, n, BLCO_F64, val, (hipStream_t)stream);
+}
+
+int blco_vecsum(double* x, uint64_t n, void* stream)
+{
+    if (!x) return fail(BLCO_EINVAL, "null vector");
+    return launch_vecsum(x, n, (hipStream_t)stream);
+}
+
+int blco_solve(double* const* grams, int nmodes, int mode, int rank, const void* M, void* U, int dtype,
+               uint64_t rows, int* used_fallback, void* stream)
+{
+    if (!grams || !M || !U || nmodes < 2 || nmodes > BLCO_MAX_MODES || mode < 0 || mode >= nmodes ||
+        rank < 1 || rank > 128)
+        return fail(BLCO_EINVAL, "bad solve arguments");
+    hipStream_t st = (hipStream_t)stream;
+    GramSet gs;
+    for (int m = 0; m < nmodes; ++m) gs.g[m] = grams[m];
+    double *fac = nullptr, *work = nullptr, *stats = nullptr;
+    int* status = nullptr;
+    uint32_t np = solve_parts(rows);
+    HIP_TRY(hipMalloc(&fac, sizeof(double) * rank * rank));
+    HIP_TRY(hipMalloc(&work, sizeof(double) * (2 * rank * rank + rank)));
+    HIP_TRY(hipMalloc(&status, sizeof(int)));
+    HIP_TRY(hipMalloc(&stats, sizeof(double) * (size_t)(np ? np : 1) * rank));
+    int rc = launch_factorize(gs, nmodes, mode, rank, fac, work, status, st);
+    if (!rc) rc = launch_solve_rows(M, U, dtype, rows, rank, fac, 0, stats, st);
+    int h_status = 0;
+    hipError_t e = hipMemcpyAsync(&h_status, status, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(fac); (void)hipFree(work); (void)hipFree(status); (void)hipFree(stats);
+    if (rc) return rc;
+    HIP_TRY(e);
+    if (used_fallback) *used_fallback = h_status;
+    return BLCO_OK;
+}
+
+int blco_colnorm(void* U, int dtype, uint64_t rows, int rank, int which, double* lambda, void* stream)
+{
+    if (!U || !lambda || rank < 1 || (which != 0 && which != 1)) return fail(BLCO_EINVAL, "bad norm arguments");
+    hipStream_t st = (hipStream_t)stream;
+    int np = (int)std::min<uint64_t>((rows + 1023) / 1024, 1024);
+    if (np < 1) np = 1;
+    double *stats = nullptr, *inv = nullptr;
+    HIP_TRY(hipMalloc(&stats, sizeof(double) * (size_t)np * rank));
+    HIP_TRY(hipMalloc(&inv, sizeof(double) * rank));
+    int rc = launch_colstats(U, dtype, rows, rank, which, stats, np, st);
+    if (!rc) rc = launch_lambda(stats, np, rank, which, lambda, inv, st);
+    if (!rc) rc = launch_scale(U, dtype, rows, rank, inv, st);
+    hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(stats); (void)hipFree(inv);
+    if (rc) return rc;
+    HIP_TRY(e);
+    return BLCO_OK;
+}
+
+int blco_fit(double* const* grams, int nmodes, int rank, const double* lambda, const void* U_last,
+             const void* M_last, int dtype, uint64_t rows, double normX2, double* fit, void* stream)
+{
+    if (!grams || !lambda || !U_last || !M_last || !fit || nmodes < 1 || nmodes > BLCO_MAX_MODES ||
+        rank < 1 || rank > 64)
+        return fail(BLCO_EINVAL, "bad fit arguments");
+    hipStream_t st = (hipStream_t)stream;
+    GramSet gs;
+    for (int m = 0; m < nmodes; ++m) gs.g[m] = grams[m];
+    int nparts;
+    uint64_t rpw;
+    gram_parts(rows, &nparts, &rpw);
+    double *part = nullptr, *G = nullptr, *ip = nullptr, *out = nullptr;
+    HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)nparts * (rank * rank + rank)));
+    HIP_TRY(hipMalloc(&G, sizeof(double) * rank * rank));
+    HIP_TRY(hipMalloc(&ip, sizeof(double) * rank));
+    HIP_TRY(hipMalloc(&out, sizeof(double) * 4));
+    int rc = launch_gram(const_cast<void*>(U_last), dtype, rows, rank, nullptr, M_last, part, G, ip, st);
+    if (!rc) rc = launch_fit(gs, nmodes, rank, lambda, ip, normX2, out, st);
+    double h[3] = {0, 0, 0};
+    hipError_t e = hipMemcpyAsync(h, out, sizeof(double) * 3, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(part); (void)hipFree(G); (void)hipFree(ip); (void)hipFree(out);
+    if (rc) return rc;
+    HIP_TRY(e);
+    *fit = h[0];
+    return BLCO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// device.hpp -- the device tensor object behind blco_dev.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <vector>
+
+#include "blco_hip.h"
+#include "mttkrp.hpp"
+
+namespace blco {
+
+// A contiguous chunk range of one block that fits one streaming slot.
+struct StreamUnit {
+    uint32_t c0, c1;
+    uint64_t begin;  // first nonzero
+    uint64_t nnz;
+};
+
+struct ChunkTab {
+    int tpw = 0;
+    Chunk* d = nullptr;
+    uint32_t n = 0;
+    std::vector<uint32_t> first;  // first chunk of each block (nblocks + 1)
+    std::vector<StreamUnit> units;
+};
+
+}  // namespace blco
+
+struct blco_dev {
+    int nmodes = 0;
+    uint64_t dims[BLCO_MAX_MODES] = {};
+    uint64_t masks[BLCO_MAX_MODES] = {};
+    int pos[BLCO_MAX_MODES] = {};
+    uint64_t nnz = 0;
+    int vdtype = BLCO_F64;
+    int stream_data = 0;
+    uint64_t slot_nnz = 0;
+    uint64_t nblocks = 0;
+    std::vector<uint64_t> bstart, bnnz;
+    uint64_t* d_bcoords = nullptr;
+    uint64_t* d_coords = nullptr;
+    void* d_vals = nullptr;
+    std::list<blco::ChunkTab> tabs;
+    // streaming
+    uint64_t* h_coords = nullptr;
+    void* h_vals = nullptr;
+    uint64_t* slot_coords[2] = {nullptr, nullptr};
+    void* slot_vals[2] = {nullptr, nullptr};
+    hipEvent_t ev_copied[2] = {nullptr, nullptr};
+    hipEvent_t ev_done[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    // lvl3 partial matrices
+    void* partials = nullptr;
+    size_t partials_bytes = 0;
+    hipStream_t stream = nullptr;
+    uint64_t bytes = 0;
+    int device = 0;
+};
+
+namespace blco {
+
+const ChunkTab* chunk_table(blco_dev* d, int tpw);
+int auto_kernel(const blco_dev* d, int mode, int rank, int ftype);
+int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
+           int nprtn, int tpw, hipStream_t st, int* used_kernel);
+
+}  // namespace blco

@@ -1,0 +1,58 @@
+// internal.hpp -- host-side objects behind the C ABI (blco_host.h).
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "blco_host.h"
+
+namespace blco {
+
+// Sets the thread's last-error message and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// 64 - clz(d - 1); a mode of length 1 needs 0 bits (the reference evaluates
+// clz(0) there, which is undefined: include/alto.hpp:613).
+inline int mode_bits(uint64_t d) { return d <= 1 ? 0 : 64 - __builtin_clzll(d - 1); }
+
+struct Layout {
+    int nmodes = 0;
+    uint64_t dims[BLCO_MAX_MODES] = {};
+    int bits[BLCO_MAX_MODES] = {};
+    int total_bits = 0;
+    // ALTO interleave masks (setup_packed_alto, include/alto.hpp:597-695)
+    uint64_t alto_lo[BLCO_MAX_MODES] = {};
+    uint64_t alto_hi[BLCO_MAX_MODES] = {};
+    // BLCO re-encoding (include/blco.hpp:222-233)
+    int trunc[BLCO_MAX_MODES] = {};
+    uint64_t masks[BLCO_MAX_MODES] = {};
+    int pos[BLCO_MAX_MODES] = {};
+};
+
+// Computes the ALTO / BLCO layout of a shape; returns BLCO_OK or an error.
+int make_layout(int nmodes, const uint64_t* dims, Layout* L);
+
+}  // namespace blco
+
+struct blco_coo {
+    int nmodes = 0;
+    uint64_t dims[BLCO_MAX_MODES] = {};
+    uint64_t nnz = 0;
+    std::vector<uint64_t> cidx;  // mode-major
+    std::vector<double> vals;
+};
+
+struct blco_host {
+    blco::Layout L;
+    uint64_t nnz = 0;
+    uint64_t max_block_size = 0;
+    uint64_t max_block_nnz = 0;
+    std::vector<uint64_t> coords;
+    std::vector<double> vals;
+    std::vector<uint64_t> block_start;
+    std::vector<uint64_t> block_nnz;
+    std::vector<uint64_t> block_coords;  // nblocks x nmodes
+};

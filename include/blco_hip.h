@@ -1,0 +1,176 @@
+/*
+ * blco_hip.h -- C ABI of the MI355X (gfx950) side of the BLCO CP-ALS path:
+ * device tensors, MTTKRP kernels, the ALS step and the CP-ALS driver.
+ *
+ * Status codes, blco_last_error() and the host tensor handle are declared in
+ * blco_host.h.  Device pointers are plain `void*` / `double*` into device
+ * memory; `stream` is a hipStream_t passed as void* (NULL = the library's own
+ * stream for that object).  Every entry point checks its arguments on the
+ * host before any kernel launch and returns an error instead of launching.
+ */
+#ifndef BLCO_HIP_H_
+#define BLCO_HIP_H_
+
+#include <stdint.h>
+
+#include "blco_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types of values / factors. */
+enum blco_dtype { BLCO_F64 = 0, BLCO_F32 = 1, BLCO_BF16 = 2 };
+
+/* Kernel ids, the reference's -k values (src/main.cpp:88,
+ * src/alto_dev.cu:1954-2032). */
+enum blco_kernel {
+    BLCO_K_LVL1 = 1,       /* wave64 segmented reduction, one launch per block */
+    BLCO_K_LVL2 = 2,       /* accepted, runs lvl1 */
+    BLCO_K_LVL3 = 3,       /* LDS-privatised output + nprtn partial matrices */
+    BLCO_K_AUTO = 10,      /* the reference heuristic */
+    BLCO_K_BATCHED = 11,   /* lvl1, all blocks in one launch */
+    BLCO_K_BATCHED2 = 12,  /* same as 11 (work list) */
+    BLCO_K_ATOMIC = 13,    /* one HBM atomic per (nonzero, column) */
+    BLCO_K_SELECT = 20     /* per-mode choice by the installed selector */
+};
+
+/* Number of visible GPUs (0 when none); never initialises a context on a
+ * machine without /dev/kfd. */
+int blco_device_count(void);
+
+/* ---------------------------------------------------------- device tensor */
+
+typedef struct blco_dev blco_dev;
+
+/* gen_blcotensor_device + send_blcotensor_over + send_masks_over
+ * (src/blco.cu:12-58, :120-138; src/alto_dev.cu:1849-1853).  value_dtype is
+ * BLCO_F64 or BLCO_F32 (values converted on upload).  stream_data = 0 keeps
+ * every block resident in HBM; 1 keeps the tensor in pinned host memory and
+ * streams blocks of at most slot_nnz nonzeros through two device slots per
+ * MTTKRP (double-buffered copies on a side stream). */
+int blco_dev_create(const blco_host* h, int value_dtype, int stream_data,
+                    uint64_t slot_nnz, blco_dev** out);
+/* Same from plain arrays (a BLCO tensor built elsewhere). */
+int blco_dev_create_arrays(int nmodes, const uint64_t* dims,
+                           const uint64_t* masks, const int* pos, uint64_t nnz,
+                           const uint64_t* coords, const double* vals,
+                           uint64_t nblocks, const uint64_t* block_start,
+                           const uint64_t* block_nnz,
+                           const uint64_t* block_coords, int value_dtype,
+                           int stream_data, uint64_t slot_nnz, blco_dev** out);
+/* delete_blcotensor_device, src/blco.cu:74-95 */
+void blco_dev_free(blco_dev* d);
+/* Bytes of device memory held by the tensor. */
+uint64_t blco_dev_bytes(const blco_dev* d);
+
+/* ----------------------------------------------------------------- MTTKRP */
+
+/* mttkrp_alto_dev_onemode, src/alto_dev.cu:1954-2032:
+ * out[i_mode, :] += sum over nonzeros of val * prod_{m != mode} U_m[i_m, :].
+ * factors: nmodes device pointers (row-major dims[m] x rank) of factor_dtype
+ * (BLCO_F64, BLCO_F32 or BLCO_BF16); out: device dims[mode] x rank, f64 for
+ * BLCO_F64 factors, f32 otherwise; it is accumulated into (the caller
+ * zeroes it, src/cpd_gpu.cu:155).  nprtn: lvl3 partial-matrix count (0 =
+ * default); thread_cf: tiles per wave (0 = default). */
+int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype,
+                void* const* factors, void* out, int kernel, int nprtn,
+                int thread_cf, void* stream);
+
+/* Kernel the AUTO heuristic would run for (mode, rank)
+ * (src/alto_dev.cu:1975,1996-2001). */
+int blco_auto_kernel(const blco_dev* d, int mode, int rank);
+
+/* --------------------------------------------------------------- ALS step */
+
+/* ata_gpu, src/cpd_gpu.cu:547-568: G = U^T U (rank x rank, f64, row-major)
+ * on MFMA from a rows x rank factor of dtype (f64 via f64 MFMA, f32/bf16 via
+ * f32 MFMA with f64 partial sums). */
+int blco_gram(const void* U, int dtype, uint64_t rows, int rank, double* G,
+              void* stream);
+/* vector_hadamard, src/cpd_gpu.cu:397-404: x <- x .* y (f64, n entries). */
+int blco_hadamard(double* x, const double* y, uint64_t n, void* stream);
+/* value_fill, src/cpd_gpu.cu:408-415 */
+int blco_value_fill(double* x, uint64_t n, double val, void* stream);
+/* vecsum, src/cpd_gpu.cu:419-448: x[0] <- sum(x[0..n)); rest is scratch. */
+int blco_vecsum(double* x, uint64_t n, void* stream);
+
+/* Normal-equation solve of one ALS mode update, replacing the reference's
+ * pseudoinverse_gpu + Dgemm (src/cpd_gpu.cu:160-177, :452-532) with the CPU
+ * reference's Cholesky / minimum-norm fallback (include/cpd.hpp:547-648):
+ * V = Hadamard of grams[m] for m != mode (nmodes device pointers, f64
+ * rank x rank); U = M V^{-1} row by row (Cholesky, forward/backward
+ * substitution), or M V^+ when V is not positive definite (symmetric
+ * pseudo-inverse, tolerance rank * ulp(max |eig|)).  M (rows x rank, f64 or
+ * f32 per dtype) is read, U (same dtype) written.  *used_fallback (host, may
+ * be NULL) receives 1 when the pseudo-inverse was used (this reads back one
+ * flag and synchronises the stream). */
+int blco_solve(double* const* grams, int nmodes, int mode, int rank,
+               const void* M, void* U, int dtype, uint64_t rows,
+               int* used_fallback, void* stream);
+
+/* Column normalisation of one factor (KruskalModelNorm,
+ * src/kruskal_model.cpp:142-259; columnwise_normscal, src/cpd_gpu.cu:244-304
+ * without the lambda carry-over): which 0 = L2 (lambda_r = ||U_:r||, zero
+ * columns left at zero), 1 = max (lambda_r = max(1, max_i U_ir)).  U scaled
+ * in place; lambda (device f64, rank) overwritten. */
+int blco_colnorm(void* U, int dtype, uint64_t rows, int rank, int which,
+                 double* lambda, void* stream);
+
+/* kruskal_fit_gpu, src/cpd_gpu.cu:333-384 with the CPU reference's residual
+ * rule (include/cpd.hpp:190-272): fit = 1 - r / ||X||, r^2 = ||X||^2 +
+ * |sum_ij lambda_i lambda_j prod_n G_n(i,j)| - 2 sum_r lambda_r
+ * sum_i U_ir M_ir (r taken as is when r^2 <= 0).  U_last / M_last are the
+ * last updated factor and its MTTKRP (rows x rank, dtype).  Result to *fit
+ * (host; synchronises the stream). */
+int blco_fit(double* const* grams, int nmodes, int rank, const double* lambda,
+             const void* U_last, const void* M_last, int dtype, uint64_t rows,
+             double normX2, double* fit, void* stream);
+
+/* ---------------------------------------------------------------- CP-ALS */
+
+/* Options of cpals_blco_dev (include/cpd_gpu.hpp:22-23). */
+typedef struct blco_cpals_opts {
+    int rank;
+    int max_iters;
+    double tolerance;   /* stop when i > 0 and |fit - fit_old| < tolerance */
+    int kernel;         /* enum blco_kernel */
+    int precision;      /* BLCO_F64 (plain form) or BLCO_F32 / BLCO_BF16 factors */
+    int nprtn;          /* lvl3 partial matrices (0 = default) */
+    int thread_cf;      /* tiles per wave (0 = default) */
+    int mode_order;     /* 0: modes 0..N-1 (CPU reference), 1: longest first (GPU reference) */
+    int use_graph;      /* 1: capture one ALS sweep in a hipGraph and replay it */
+    int verbose;        /* 1: print the reference's per-iteration lines */
+} blco_cpals_opts;
+
+typedef struct blco_cpals_state blco_cpals_state; /* device-resident solver state */
+
+/* Uploads the initial factors U[n] (host f64, dims[n] x rank) and prepares
+ * the solver: grams, work buffers, ||X||^2. */
+int blco_cpals_create(blco_dev* d, const blco_cpals_opts* opts,
+                      const double* const* U, blco_cpals_state** out);
+/* Runs up to n_iters ALS iterations (stopping early on the tolerance test),
+ * appending one fit per iteration to fits (host, may be NULL); returns the
+ * number of iterations run in *done. */
+int blco_cpals_iterate(blco_cpals_state* s, int n_iters, double* fits, int* done);
+/* Factors (dims[n] x rank, converted to f64) and lambda back to the host. */
+int blco_cpals_get(blco_cpals_state* s, double* const* U, double* lambda);
+/* Accumulated device time of the MTTKRP launches per mode (ms, nmodes
+ * entries, measured with HIP events on the solver stream), of whole
+ * iterations, the iteration count, and the kernel chosen per mode. */
+int blco_cpals_times(const blco_cpals_state* s, double* mttkrp_ms, double* iter_ms,
+                     int* iters, int* kernels);
+void blco_cpals_reset_times(blco_cpals_state* s);
+void blco_cpals_free(blco_cpals_state* s);
+
+/* cpals_blco_dev, src/cpd_gpu.cu:21-242, in one call: create, iterate up to
+ * max_iters, copy back.  fit_trace (host, max_iters entries, may be NULL);
+ * returns iterations run in *iters and the final fit in *final_fit. */
+int blco_cpals(blco_dev* d, const blco_cpals_opts* opts, double* const* U,
+               double* lambda, double* fit_trace, int* iters,
+               double* final_fit);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

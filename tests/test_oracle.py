@@ -1,0 +1,349 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip in the restatement (wrong
+dealing order, dropped term, transposed operand, wrong norm) fails: the
+compiled reference sources (oracle/_ref, when present), hand-worked masks,
+brute-force definitions and closed forms."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, random_coo
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def py_pdep(x, m):
+    r, b = 0, 0
+    for i in range(128):
+        if (m >> i) & 1:
+            if (x >> b) & 1:
+                r |= 1 << i
+            b += 1
+    return r
+
+
+def py_pext(x, m):
+    r, b = 0, 0
+    for i in range(128):
+        if (m >> i) & 1:
+            if (x >> i) & 1:
+                r |= 1 << b
+            b += 1
+    return r
+
+
+def test_pdep_pext_64_bruteforce(O):
+    L = O.lib()
+    rng = np.random.default_rng(11)
+    for _ in range(400):
+        x = int(rng.integers(0, 2**63)) * 2 + int(rng.integers(0, 2))
+        m = int(rng.integers(0, 2**63)) ^ int(rng.integers(0, 2**63)) << 1
+        m &= (1 << 64) - 1
+        assert L.orc_pdep64(x, m) == py_pdep(x, m) & ((1 << 64) - 1)
+        assert L.orc_pext64(x, m) == py_pext(x, m)
+        k = bin(m).count("1")
+        assert L.orc_pext64(L.orc_pdep64(x, m), m) == x & ((1 << k) - 1)
+
+
+def test_pdep_pext_128_bruteforce(O):
+    import ctypes as C
+    L = O.lib()
+    rng = np.random.default_rng(12)
+    for _ in range(300):
+        mlo = int(rng.integers(0, 2**63))
+        mhi = int(rng.integers(0, 2**20))
+        x = int(rng.integers(0, 2**62))
+        lo, hi = C.c_uint64(), C.c_uint64()
+        L.orc_pdep128(x, mlo, mhi, C.byref(lo), C.byref(hi))
+        full = py_pdep(x, (mhi << 64) | mlo)
+        assert lo.value == full & ((1 << 64) - 1) and hi.value == full >> 64
+        k = bin(mlo).count("1") + bin(mhi).count("1")
+        assert L.orc_pext128(lo.value, hi.value, mlo, mhi) == x & ((1 << k) - 1)
+
+
+def test_alto_masks_golden(O):
+    cases = json.load(open(os.path.join(GOLD, "alto_masks.json")))["cases"]
+    for c in cases:
+        lo, hi, pos, bits = O.alto_setup(c["dims"], 128)
+        got = [hex((int(h) << 64) | int(l)) for l, h in zip(lo, hi)]
+        assert got == c["alto_masks"], c["dims"]
+        assert list(pos) == c["mode_pos"] and list(bits) == c["bits"]
+        masks = [(int(h) << 64) | int(l) for l, h in zip(lo, hi)]
+        union = 0
+        for m in masks:
+            assert union & m == 0  # disjoint
+            union |= m
+        assert union == (1 << c["total_bits"]) - 1
+        assert [bin(m).count("1") for m in masks] == c["bits"]
+
+
+def test_alto_setup_matches_reference(O):
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    rng = np.random.default_rng(3)
+    shapes = [(2000, 1500, 1000), (12092, 9184, 28818), (7, 7, 7, 7), (1 << 20, 3, 1000, 17),
+              (5, 100000, 33, 2, 9), (4800000, 1800000, 1800000)]
+    for _ in range(30):
+        n = int(rng.integers(2, 6))
+        shapes.append(tuple(int(x) for x in rng.integers(2, 1 << 18, n)))
+    for dims in shapes:
+        lit = 128 if sum(int(d - 1).bit_length() for d in dims) > 64 else 64
+        lo, hi, pos, _ = O.alto_setup(dims, lit)
+        rlo, rhi, rpos = O.ref_alto_setup(dims, lit)
+        assert (lo == rlo).all() and (hi == rhi).all() and (pos == rpos).all(), dims
+
+
+@pytest.mark.parametrize("dims,lit", [((300, 200, 50), 64), ((40, 4000, 5, 600), 64),
+                                      ((1 << 22, 1 << 22, 1 << 21), 128)])
+def test_alto_encode_matches_reference(O, dims, lit):
+    rng = np.random.default_rng(5)
+    c, v = random_coo(rng, dims, 3000)
+    klo, khi, vo, perm = O.alto_encode(dims, c, v, lit)
+    key = (khi.astype(object) << 64) | klo.astype(object)
+    assert all(key[i] < key[i + 1] for i in range(len(key) - 1))  # strictly increasing
+    assert (vo == v[perm]).all()
+    lo, hi, pos, _ = O.alto_setup(dims, lit)
+    for m in range(len(dims)):
+        mm = (int(hi[m]) << 64) | int(lo[m])
+        for i in range(0, c.shape[1], 97):
+            assert py_pext(int(key[i]), mm) == int(c[m, perm[i]])
+    if O.ref_lib() is not None:
+        rk, rh, rv = O.ref_gen_alto(dims, c, v, lit)
+        assert (rk == klo).all() and (rh == khi).all() and (rv == vo).all()
+
+
+@pytest.mark.parametrize("dims,mbs", [((300, 200, 50), 0), ((300, 200, 50), 700),
+                                      ((1 << 22, 1 << 22, 1 << 21), 0),
+                                      ((1 << 22, 1 << 22, 1 << 21), 333),
+                                      ((1 << 30, 1 << 30, 1 << 30, 1 << 20), 0)])
+def test_blco_encode_invariants(O, dims, mbs):
+    rng = np.random.default_rng(7)
+    c, v = random_coo(rng, dims, 4000)
+    b = O.blco_encode(dims, c, v, mbs, 128)
+    klo, khi, vo, perm = O.alto_encode(dims, c, v, 128)
+    # concatenation of blocks is the ALTO order, values follow
+    assert (b["vals"] == vo).all()
+    starts, sizes = b["block_start"], b["block_nnz"]
+    assert starts[0] == 0 and (starts[1:] == starts[:-1] + sizes[:-1]).all()
+    assert sizes.sum() == c.shape[1]
+    if mbs:
+        assert sizes.max() <= mbs
+    # one upper-key value per block
+    for s, n in zip(starts, sizes):
+        assert len(set(khi[s:s + n].tolist())) == 1
+    # decode(coords, block_coords) == original coordinate
+    blk = np.repeat(np.arange(len(sizes)), sizes.astype(np.int64))
+    for m in range(len(dims)):
+        dec = ((b["coords"] >> np.uint64(b["pos"][m])) & b["masks"][m]) | b["block_coords"][blk, m]
+        assert (dec == c[m, perm]).all()
+    # masks / positions follow the truncated low-half bit counts
+    lo, hi, _, _ = O.alto_setup(dims, 128)
+    trunc = [bin(int(x)).count("1") for x in lo]
+    assert list(b["pos"]) == [sum(trunc[:m]) for m in range(len(dims))]
+    assert [int(x) for x in b["masks"]] == [(1 << t) - 1 for t in trunc]
+
+
+def test_blco_golden_65bit_layout(O):
+    case = json.load(open(os.path.join(GOLD, "alto_masks.json")))["cases"][2]
+    dims = case["dims"]
+    c = np.array([[0, dims[0] - 1, 123456, 4194304], [0, dims[1] - 1, 77, 5],
+                  [0, dims[2] - 1, 999, 6]], np.uint64)
+    b = O.blco_encode(dims, c, np.ones(4), 0, 128)
+    assert list(b["pos"]) == case["blco_pos"]
+    assert [bin(int(m)).count("1") for m in b["masks"]] == case["blco_trunc"]
+    assert len(b["block_nnz"]) == 2  # index 4194304 and dims[0]-1 need bit 64
+    assert sorted(b["block_coords"][:, 0].tolist()) == [0, 1 << 22]
+
+
+def dense_of(dims, c, v):
+    X = np.zeros(dims)
+    X[tuple(c.astype(np.int64))] = v
+    return X
+
+
+def mttkrp_dense(X, U, mode):
+    N = X.ndim
+    letters = "abcde"[:N]
+    ins = [letters] + [letters[m] + "r" for m in range(N) if m != mode]
+    ops = [X] + [U[m] for m in range(N) if m != mode]
+    return np.einsum(",".join(ins) + "->" + letters[mode] + "r", *ops)
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 4), (3, 4, 5, 6), (4, 3, 2, 3, 2)])
+def test_mttkrp_coo_dense(O, dims):
+    rng = np.random.default_rng(21)
+    c, v = random_coo(rng, dims, 40)
+    R = 5
+    U = [rng.random((d, R)) for d in dims]
+    X = dense_of(dims, c, v)
+    for m in range(len(dims)):
+        assert np.allclose(O.mttkrp_coo(c, v, U, m), mttkrp_dense(X, U, m), rtol=1e-13, atol=1e-13)
+
+
+def test_mttkrp_closed_forms(O):
+    rng = np.random.default_rng(22)
+    dims = (30, 20, 10)
+    c, v = random_coo(rng, dims, 500)
+    ones = [np.ones((d, 3)) for d in dims]
+    for m in range(3):
+        slice_sum = np.bincount(c[m].astype(np.int64), weights=v, minlength=dims[m])
+        got = O.mttkrp_coo(c, v, ones, m)
+        assert np.allclose(got, slice_sum[:, None] * np.ones((1, 3)), rtol=1e-13)
+    # rank-1 separable values: v = a_i b_j c_k -> MTTKRP mode 0 with U=(.,b,c)
+    a, bb, cc = [rng.random(d) + 0.1 for d in dims]
+    vv = a[c[0].astype(int)] * bb[c[1].astype(int)] * cc[c[2].astype(int)]
+    U = [np.ones((dims[0], 1)), bb[:, None], cc[:, None]]
+    got = O.mttkrp_coo(c, vv, U, 0)[:, 0]
+    want = np.zeros(dims[0])
+    np.add.at(want, c[0].astype(int), vv * bb[c[1].astype(int)] * cc[c[2].astype(int)])
+    assert np.allclose(got, want, rtol=1e-13)
+
+
+@pytest.mark.parametrize("dims,nnz", [((60, 50, 40), 20000), ((3000, 2000, 1000), 4000),
+                                      ((20, 30, 40, 10), 5000), ((9, 8, 7, 6, 5), 3000)])
+def test_alto_mttkrp_matches_coo_and_reference(O, dims, nnz):
+    rng = np.random.default_rng(23)
+    c, v = random_coo(rng, dims, nnz)
+    R = 8
+    U = [rng.random((d, R)) for d in dims]
+    at = O.Alto(dims, c, v, nprtn=4)
+    for m in range(len(dims)):
+        want = O.mttkrp_coo(c, v, U, m)
+        got = at.mttkrp(U, m)
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+        if O.ref_lib() is not None and len(dims) <= 5:
+            ref = O.ref_mttkrp(dims, c, v, U, m, 4)
+            assert np.linalg.norm(ref - want) <= 1e-13 * np.linalg.norm(want)
+
+
+def test_cholesky_and_solve(O):
+    rng = np.random.default_rng(31)
+    for R in (1, 3, 16, 32):
+        A = rng.random((R + 5, R))
+        V = A.T @ A + 0.1 * np.eye(R)
+        info, L = O.cholesky(V)
+        assert info == 0
+        assert np.allclose(np.tril(L), L) and np.allclose(L @ L.T, V, rtol=1e-12, atol=1e-12)
+        B = rng.random((7, R))
+        X = O.cholesky_solve_rows(L, B)
+        assert np.allclose(X @ V, B, rtol=1e-9, atol=1e-9)
+    V = np.ones((4, 4))
+    info, _ = O.cholesky(V)
+    assert info == 2  # second pivot is zero
+
+
+def test_pinv_sym(O):
+    rng = np.random.default_rng(32)
+    for R, k in ((6, 3), (16, 15), (32, 32), (8, 1)):
+        A = rng.random((k, R))
+        V = A.T @ A
+        P = O.pinv_sym(V)
+        assert np.allclose(V @ P @ V, V, atol=1e-9 * np.abs(V).max())
+        assert np.allclose(P @ V @ P, P, atol=1e-6 * np.abs(P).max())
+        assert np.allclose(V @ P, (V @ P).T, atol=1e-9)
+        assert np.allclose(P, np.linalg.pinv(V, hermitian=True), rtol=1e-6,
+                           atol=1e-8 * np.abs(P).max())
+
+
+def test_kruskal_norm(O):
+    rng = np.random.default_rng(33)
+    U = rng.random((50, 6)) * 3 - 0.5
+    A, la = O.kruskal_norm(U, 0)
+    assert np.allclose(np.linalg.norm(A, axis=0), 1.0)
+    assert np.allclose(A * la, U)
+    B, lb = O.kruskal_norm(U, 1)
+    assert np.allclose(lb, np.maximum(1.0, U.max(axis=0)))
+    assert np.allclose(B * lb, U)
+    if O.ref_lib() is not None:
+        for which in (0, 1):
+            X, lx = O.kruskal_norm(U, which)
+            Y, ly = O.ref_kruskal_norm(U, which)
+            assert np.allclose(X, Y, rtol=1e-14) and np.allclose(lx, ly, rtol=1e-14)
+
+
+def test_fit_closed_form(O):
+    rng = np.random.default_rng(34)
+    dims = (7, 6, 5)
+    c, v = random_coo(rng, dims, 60)
+    X = dense_of(dims, c, v)
+    R = 3
+    U = [rng.random((d, R)) for d in dims]
+    lam = rng.random(R) + 0.5
+    grams = [u.T @ u for u in U]
+    M = mttkrp_dense(X, U, 2)
+    f = O.cpd_fit(U[2], M, lam, grams, float((v * v).sum()))
+    K = np.einsum("r,ir,jr,kr->ijk", lam, U[0], U[1], U[2])
+    want = 1 - np.linalg.norm(X - K) / np.linalg.norm(X)
+    assert abs(f - want) < 1e-12
+
+
+def numpy_als_sweep(dims, c, v, U, it):
+    """One cpd_alto sweep restated with numpy (solve by numpy.linalg)."""
+    X = dense_of(dims, c, v)
+    U = [u.copy() for u in U]
+    N = len(dims)
+    lam = None
+    for n in range(N):
+        M = mttkrp_dense(X, U, n)
+        V = np.ones((U[0].shape[1],) * 2)
+        for m in range(N):
+            if m != n:
+                V = V * (U[m].T @ U[m])
+        Un = np.linalg.solve(V, M.T).T
+        if it == 0:
+            lam = np.linalg.norm(Un, axis=0)
+        else:
+            lam = np.maximum(1.0, Un.max(axis=0))
+        U[n] = Un / lam
+    return U, lam
+
+
+def test_cpd_one_sweep_matches_numpy(O):
+    rng = np.random.default_rng(35)
+    dims = (12, 10, 8)
+    c, v = random_coo(rng, dims, 300)
+    R = 4
+    U0 = [rng.random((d, R)) for d in dims]
+    at = O.Alto(dims, c, v, nprtn=3)
+    U1, lam1, fits, _, _ = at.cpd(U0, 1)
+    Uw, lamw = numpy_als_sweep(dims, c, v, U0, 0)
+    for a, b in zip(U1, Uw):
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
+    assert np.allclose(lam1, lamw, rtol=1e-10)
+    U2, lam2, fits2, _, _ = at.cpd(U0, 2)
+    Uw2, lamw2 = numpy_als_sweep(dims, c, v, Uw, 1)
+    for a, b in zip(U2, Uw2):
+        assert np.allclose(a, b, rtol=1e-8, atol=1e-12)
+    assert np.allclose(lam2, lamw2, rtol=1e-9)
+
+
+def test_cpd_exact_low_rank_converges(O):
+    rng = np.random.default_rng(36)
+    dims = (15, 12, 10)
+    R = 2
+    A = [rng.random((d, R)) + 0.2 for d in dims]
+    X = np.einsum("ir,jr,kr->ijk", *A)
+    c = np.array(np.nonzero(X >= 0)).astype(np.uint64)  # dense, every cell
+    v = X[tuple(c.astype(np.int64))]
+    at = O.Alto(dims, c, v, nprtn=2)
+    U0 = [rng.random((d, R)) for d in dims]
+    _, _, fits, _, _ = at.cpd(U0, 200, 0.0)
+    assert fits[-1] > 0.999
+
+
+def test_cpd_singular_normal_matrix_uses_pinv(O):
+    """Two identical initial columns make V singular: the Cholesky fails and
+    the minimum-norm fallback must still give a finite decomposition."""
+    rng = np.random.default_rng(37)
+    dims = (20, 15, 10)
+    c, v = random_coo(rng, dims, 600)
+    R = 3
+    U0 = [rng.random((d, R)) for d in dims]
+    for u in U0:
+        u[:, 1] = u[:, 0]
+    at = O.Alto(dims, c, v, nprtn=2)
+    U, lam, fits, _, _ = at.cpd(U0, 3)
+    assert np.isfinite(fits).all() and all(np.isfinite(u).all() for u in U)
