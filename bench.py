@@ -1,0 +1,183 @@
+"""Benchmark: CP-ALS seconds per iteration and MTTKRP HBM GB/s on one MI355X,
+FP32 form (f32 factors/values, f64 solve/lambda/fit) against the plain FP64
+form, on the 12092 x 9184 x 28818 tensor with 77M nonzeros (modes 0,1
+Zipf(1.1), mode 2 uniform, log1p Poisson values from a rank-20 Gamma(1,1)
+model), rank 32.
+
+A step is one CP-ALS iteration (MTTKRP + solve + normalisation + Gram for
+every mode, then the fit).  The tensor is generated, encoded to BLCO and
+uploaded before timing; W warm-up iterations run untimed, then K iterations
+are timed with HIP events on the solver stream (and a host clock around
+them).  MTTKRP kernel time per mode is event-timed on the same stream.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CP-ALS s/iter + MTTKRP HBM GB/s: FP32+learned selector vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: dims, nnz, zipf, gen_rank, value_kind, rank, seed (1000 + config)
+    "nell2-shape-77M": dict(dims=(12092, 9184, 28818), nnz=77_000_000, zipf=(1.1, 1.1, 0.0),
+                            gen_rank=20, value_kind=1, rank=32, seed=1002),
+    "small-1M": dict(dims=(2000, 1500, 1000), nnz=1_000_000, zipf=None, gen_rank=10,
+                     value_kind=0, rank=16, seed=1001),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(dims, nnz, rank, s_v, s_f, mode):
+    """Compulsory MTTKRP traffic of one mode: stream the tensor once (8-byte
+    coordinate + value), read each input factor once, write the output once."""
+    b = nnz * (8 + s_v)
+    for m, d in enumerate(dims):
+        b += d * rank * s_f
+    return b
+
+
+def run_form(B, dev, U0, rank, precision, kernel, warmup, steps):
+    s = B.CpAls(dev, U0, rank, precision=precision, kernel=kernel)
+    f0 = s.iterate(warmup) if warmup else np.zeros(0)
+    s.reset_times()
+    B.synchronize()
+    t0 = time.perf_counter()
+    fits = s.iterate(steps)
+    B.synchronize()
+    wall = time.perf_counter() - t0
+    tm = s.times()
+    s.free()
+    return dict(fits=np.concatenate([f0, fits]), wall=wall, **tm)
+
+
+def cpu_baseline(t, dims, rank, U0, iters, log_fn):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    at = O.Alto(dims, t.cidx, t.vals, nprtn=threads)
+    log_fn("cpu baseline: ALTO construction %.1f s (%d threads)" % (time.perf_counter() - t0, threads))
+    # 1 warm-up iteration then `iters` timed, from the same initial factors
+    _, _, fits, its, mts = at.cpd(U0, 1 + iters, 0.0)
+    return dict(s_per_iter=float(np.mean(its[1:])), mttkrp_s=float(np.mean(mts[1:])),
+                threads=threads, fits=fits)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="nell2-shape-77M", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as G
+    B = G._import_pkg()
+    if B.device_count() < 1:
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    W = WORKLOADS[args.workload]
+    dims, R = W["dims"], W["rank"]
+
+    t0 = time.perf_counter()
+    coo = B.Coo.generate(dims, W["nnz"], zipf=W["zipf"], gen_rank=W["gen_rank"],
+                         value_kind=W["value_kind"], seed=W["seed"], topup=W["zipf"] is not None)
+    log("generated %d nonzeros in %.1f s" % (coo.nnz, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    h = B.encode(coo)
+    log("BLCO encode %.1f s, %d block(s)" % (time.perf_counter() - t0, h.nblocks))
+    U0 = B.init_factors(W["seed"], dims, R)
+    nnz = coo.nnz
+
+    results = {}
+    forms = [("fp32", B.F32, 1)] + ([] if args.no_fp64 else [("fp64", B.F64, 0)])
+    for name, prec, vdt in forms:
+        dev = B.DeviceTensor(h, value_dtype=vdt)
+        kernel = B.K_SELECT if prec != B.F64 else B.K_AUTO
+        r = run_form(B, dev, U0, R, prec, kernel, args.warmup, args.steps)
+        dev.free()
+        s_v = 8 if vdt == 0 else 4
+        s_f = 8 if prec == B.F64 else 4
+        per_mode_ms = r["mttkrp_ms"] / max(r["iters"], 1)
+        abytes = [algorithmic_bytes(dims, nnz, R, s_v, s_f, m) for m in range(len(dims))]
+        gbs = [ab / (ms * 1e-3) / 1e9 for ab, ms in zip(abytes, per_mode_ms)]
+        r.update(per_mode_ms=per_mode_ms, abytes=abytes, mode_gbs=gbs,
+                 s_per_iter=r["iter_ms"] / max(r["iters"], 1) / 1e3,
+                 mttkrp_gbs=sum(abytes) / (sum(per_mode_ms) * 1e-3) / 1e9)
+        results[name] = r
+        log("%s: %.4f s/iter (host %.4f), MTTKRP ms/mode %s, GB/s %s, final fit %.9f" % (
+            name, r["s_per_iter"], r["wall"] / max(args.steps, 1),
+            np.round(per_mode_ms, 3).tolist(), np.round(gbs, 1).tolist(), r["fits"][-1]))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(coo, dims, R, U0, args.cpu_iters, log)
+        log("cpu baseline: %.3f s/iter (MTTKRP %.3f s)" % (cpu["s_per_iter"], cpu["mttkrp_s"]))
+
+    main_r = results["fp32"]
+    out = {
+        "metric": METRIC,
+        "value": round(main_r["s_per_iter"], 6),
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_r["s_per_iter"] * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factors/values, f64 solve/lambda/fit",
+        "data": "synthetic (seeded counter-based Poisson generator)",
+        "config": {"workload": args.workload, "dims": list(dims), "nnz": int(nnz), "rank": R,
+                   "zipf": list(W["zipf"]) if W["zipf"] else None,
+                   "kernel_choice": "heuristic (-k 10 rule; learned selector not installed)",
+                   "parallelism": "single GPU"},
+        "mttkrp_gbs_fp32": round(main_r["mttkrp_gbs"], 1),
+        "mttkrp_ms_per_mode_fp32": [round(x, 4) for x in main_r["per_mode_ms"]],
+        "fit_fp32": float(main_r["fits"][-1]),
+        "roofline": {"bound": "hbm", "achieved": round(main_r["mttkrp_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(main_r["mttkrp_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "k_mttkrp_lvl1 (all modes), algorithmic bytes "
+                               "nnz*(8+4) + sum_m I_m*R*4 per launch"},
+    }
+    if "fp64" in results:
+        r64 = results["fp64"]
+        out.update({"s_per_iter_fp64": round(r64["s_per_iter"], 6),
+                    "speedup_fp32_vs_fp64": round(r64["s_per_iter"] / main_r["s_per_iter"], 3),
+                    "mttkrp_gbs_fp64": round(r64["mttkrp_gbs"], 1),
+                    "mttkrp_ms_per_mode_fp64": [round(x, 4) for x in r64["per_mode_ms"]],
+                    "fit_fp64": float(r64["fits"][-1]),
+                    "fit_diff_fp32_vs_fp64": float(abs(main_r["fits"][-1] - r64["fits"][-1]))})
+    if cpu is not None:
+        out["cpu_baseline"] = {"value": round(cpu["s_per_iter"], 4), "unit": "s/iter",
+                               "cores": cpu["threads"], "kind": "port",
+                               "sample": "oracle ALTO CPU CP-ALS (OpenMP) on the full tensor, "
+                                         "rank 32 FP64, mean of %d iterations after 1 warm-up"
+                                         % args.cpu_iters,
+                               "mttkrp_s_per_iter": round(cpu["mttkrp_s"], 4)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,12 @@ def lib():
             "blco_host_values": (vp, [vp]),
             "blco_host_free": (None, [vp]),
             "blco_device_count": (C.c_int, []),
+            "blco_malloc": (C.c_int, [C.POINTER(vp), C.c_uint64]),
+            "blco_free": (C.c_int, [vp]),
+            "blco_memcpy_htod": (C.c_int, [vp, vp, C.c_uint64]),
+            "blco_memcpy_dtoh": (C.c_int, [vp, vp, C.c_uint64]),
+            "blco_memset_zero": (C.c_int, [vp, C.c_uint64]),
+            "blco_synchronize": (C.c_int, []),
             "blco_dev_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
             "blco_dev_create_arrays": (C.c_int, [C.c_int, u64p, u64p, i32p, C.c_uint64, u64p, f64p,
                                                  C.c_uint64, u64p, u64p, u64p, C.c_int, C.c_int,
@@ -138,7 +144,7 @@ def _ptr_array(ptrs):
 
 
 def _dptr(t):
-    """Device pointer of a torch tensor (or an int address)."""
+    """Device address of a DeviceArray (or an int address)."""
     if t is None:
         return None
     if isinstance(t, int):
@@ -147,15 +153,77 @@ def _dptr(t):
 
 
 def _stream(stream):
-    if stream is not None:
-        return stream
-    try:
-        import torch
-        if torch.cuda.is_available():
-            return torch.cuda.current_stream().cuda_stream
-    except Exception:
-        pass
-    return None
+    return stream
+
+
+def synchronize():
+    _check(lib().blco_synchronize())
+
+
+def _to_bf16_bits(a):
+    """float64 -> bfloat16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _from_bf16_bits(b):
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+_ESZ = {F64: 8, F32: 4, BF16: 2}
+
+
+class DeviceArray:
+    """A device buffer from blco_malloc holding f64, f32 or bf16 elements."""
+
+    def __init__(self, shape, dtype=F64):
+        self.shape = tuple(int(x) for x in shape)
+        self.dtype = dtype
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * _ESZ[dtype]
+        p = vp()
+        _check(lib().blco_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+        if self.nbytes:
+            _check(lib().blco_memset_zero(self.ptr, self.nbytes))
+
+    @classmethod
+    def from_numpy(cls, a, dtype=F64):
+        a = np.asarray(a, np.float64)
+        d = cls(a.shape, dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.asarray(a, np.float64).reshape(-1)
+        if self.dtype == F64:
+            h = np.ascontiguousarray(a)
+        elif self.dtype == F32:
+            h = np.ascontiguousarray(a, np.float32)
+        else:
+            h = _to_bf16_bits(a)
+        if self.nbytes:
+            _check(lib().blco_memcpy_htod(self.ptr, h.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        """Copy back as float64 (waits for all device work)."""
+        h = np.zeros(self.size, {F64: np.float64, F32: np.float32, BF16: np.uint16}[self.dtype])
+        if self.nbytes:
+            _check(lib().blco_memcpy_dtoh(h.ctypes.data, self.ptr, self.nbytes))
+        out = _from_bf16_bits(h) if self.dtype == BF16 else h.astype(np.float64)
+        return out.reshape(self.shape)
+
+    def data_ptr(self):
+        return self.ptr
+
+    def free(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.blco_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
 
 
 # ------------------------------------------------------------------ COO
@@ -344,7 +412,7 @@ class DeviceTensor:
 
 def mttkrp(dev, mode, rank, factors, out, factor_dtype=F64, kernel=K_AUTO, nprtn=0,
            thread_cf=0, stream=None):
-    """out += MTTKRP(mode); factors/out are torch CUDA tensors (or addresses)."""
+    """out += MTTKRP(mode); factors/out are DeviceArrays (or device addresses)."""
     ptrs = _ptr_array([_dptr(f) if f is not None else 0 for f in factors])
     _check(lib().blco_mttkrp(dev.h, mode, rank, factor_dtype, ptrs, _dptr(out), kernel, nprtn,
                              thread_cf, _stream(stream)))

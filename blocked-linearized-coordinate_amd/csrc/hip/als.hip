@@ -6,7 +6,9 @@
 //              v_mfma_f32_16x16x4f32 for f32/bf16 factors.  One wave walks
 //              4-row steps of U; lane l holds U[r0 + l/16][16 t + l%16], which
 //              is both the A operand (A[i][k] = U[r0+k][16 ti + i]) and the B
-//              operand of tile (ti, tj).  Per-workgroup partial Grams are
+//              operand of tile (ti, tj).  The accumulator layouts differ: the
+//              f32 form leaves D[4*(l/16) + v][l%16] in register v of lane l,
+//              the f64 form D[l/16 + 4*v][l%16] (checked on the hardware).  Per-workgroup partial Grams are
 //              summed in a fixed order (deterministic).  Optionally the same
 //              pass scales U by 1/lambda first (normalisation) and forms the
 //              fit inner products sum_i U_ir M_ir.
@@ -117,7 +119,9 @@ __global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, 
         for (int tj = ti; tj < T; ++tj) {
             #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int i = ti * 16 + 4 * k4 + v, j = tj * 16 + i16;
+                // D register v of lane l holds row 4*(l/16) + v (f32 form) or
+                // row l/16 + 4*v (f64 form), column l % 16 of the tile
+                int i = ti * 16 + (F64 ? k4 + 4 * v : 4 * k4 + v), j = tj * 16 + i16;
                 double x = (double)acc[q][v];
                 s_g[w][i][j] = x;
                 s_g[w][j][i] = x;

@@ -251,6 +251,59 @@ static bool have_kfd() { return access("/dev/kfd", F_OK) == 0; }
 
 extern "C" {
 
+int blco_malloc(void** ptr, uint64_t bytes)
+{
+    if (!ptr) return fail(BLCO_EINVAL, "null output");
+    *ptr = nullptr;
+    if (!have_kfd()) return fail(BLCO_ENODEV, "no GPU (/dev/kfd absent)");
+    if (bytes == 0) return BLCO_OK;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        *ptr = nullptr;
+        return fail(BLCO_ENOMEM, "device allocation of %llu bytes failed", (unsigned long long)bytes);
+    }
+    return BLCO_OK;
+}
+
+int blco_free(void* ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return BLCO_OK;
+}
+
+int blco_memcpy_htod(void* dst, const void* src, uint64_t bytes)
+{
+    if (bytes == 0) return BLCO_OK;
+    if (!dst || !src) return fail(BLCO_EINVAL, "null pointer");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return BLCO_OK;
+}
+
+int blco_memcpy_dtoh(void* dst, const void* src, uint64_t bytes)
+{
+    if (bytes == 0) return BLCO_OK;
+    if (!dst || !src) return fail(BLCO_EINVAL, "null pointer");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BLCO_OK;
+}
+
+int blco_memset_zero(void* ptr, uint64_t bytes)
+{
+    if (bytes == 0) return BLCO_OK;
+    if (!ptr) return fail(BLCO_EINVAL, "null pointer");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemset(ptr, 0, bytes));
+    HIP_TRY(hipDeviceSynchronize());
+    return BLCO_OK;
+}
+
+int blco_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return BLCO_OK;
+}
+
 int blco_device_count(void)
 {
     if (!have_kfd()) return 0;

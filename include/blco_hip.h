@@ -39,6 +39,15 @@ enum blco_kernel {
  * machine without /dev/kfd. */
 int blco_device_count(void);
 
+/* Device memory for callers without their own allocator (plain hipMalloc /
+ * hipMemcpy on the null stream; the copies wait for all prior device work). */
+int blco_malloc(void** ptr, uint64_t bytes);
+int blco_free(void* ptr);
+int blco_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int blco_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int blco_memset_zero(void* ptr, uint64_t bytes);
+int blco_synchronize(void);
+
 /* ---------------------------------------------------------- device tensor */
 
 typedef struct blco_dev blco_dev;

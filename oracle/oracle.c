@@ -364,7 +364,7 @@ struct orc_alto {
     uint64_t* prtn_ptr;            /* nprtn + 1 */
     orc_interval* intervals;       /* nprtn * nmode */
     double** ofibs;                /* da_mem buffers, one per partition */
-    uint64_t* ofib_rows;           /* capacity (rows) of each buffer */
+    uint64_t* ofib_rows;           /* capacity (rows x rank cells) of each buffer */
 };
 
 uint64_t orc_alto_nnz(const orc_alto* at) { return at->nnz; }
@@ -499,10 +499,11 @@ static void alto_mttkrp_pull(orc_alto* at, int mode, int rank,
     for (int p = 0; p < np; ++p) {
         orc_interval iv = at->intervals[(size_t)p * nm + mode];
         uint64_t rows = (iv.stop >= iv.start) ? iv.stop - iv.start + 1 : 0;
-        if (rows > at->ofib_rows[p]) {
+        uint64_t cells = rows * (uint64_t)rank;  /* capacity is kept in cells */
+        if (cells > at->ofib_rows[p]) {
             free(at->ofibs[p]);
-            at->ofibs[p] = (double*)malloc(sizeof(double) * rows * (uint64_t)rank);
-            at->ofib_rows[p] = rows;
+            at->ofibs[p] = (double*)malloc(sizeof(double) * cells);
+            at->ofib_rows[p] = cells;
         }
     }
     uint64_t I = at->dims[mode];

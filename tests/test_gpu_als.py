@@ -11,38 +11,33 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def dev_array(torch, a, dtype=0):
-    t = torch.from_numpy(np.ascontiguousarray(a))
-    if dtype == 1:
-        t = t.float()
-    elif dtype == 2:
-        t = t.float().to(torch.bfloat16)
-    return t.cuda()
+def dev_array(B, a, dtype=0):
+    return B.DeviceArray.from_numpy(a, dtype)
 
 
 @pytest.mark.parametrize("rank", [1, 8, 16, 20, 32, 48, 64])
 @pytest.mark.parametrize("rows", [1, 7, 1000, 28818])
-def test_gram_f64(B, torch_cuda, rank, rows):
+def test_gram_f64(B, gpu, rank, rows):
     rng = np.random.default_rng(rank * 7 + rows)
     U = rng.random((rows, rank)) - 0.3
-    G = torch_cuda.zeros((rank, rank), dtype=torch_cuda.float64, device="cuda")
-    B.gram(dev_array(torch_cuda, U), 0, rows, rank, G)
+    G = B.DeviceArray((rank, rank))
+    B.gram(dev_array(B, U), 0, rows, rank, G)
     want = U.T @ U
-    got = G.cpu().numpy()
+    got = G.numpy()
     assert np.abs(got - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
     assert (got == got.T).all()
 
 
 @pytest.mark.parametrize("dtype,tol", [(1, 2e-6), (2, 1e-5)])
-def test_gram_f32_bf16(B, torch_cuda, dtype, tol):
+def test_gram_f32_bf16(B, gpu, dtype, tol):
     rng = np.random.default_rng(3)
     U = rng.random((5000, 32))
-    Ud = dev_array(torch_cuda, U, dtype)
-    Uq = Ud.double().cpu().numpy()  # what the device holds
-    G = torch_cuda.zeros((32, 32), dtype=torch_cuda.float64, device="cuda")
+    Ud = dev_array(B, U, dtype)
+    Uq = Ud.numpy()  # what the device holds
+    G = B.DeviceArray((32, 32))
     B.gram(Ud, dtype, 5000, 32, G)
     want = Uq.T @ Uq
-    assert np.abs(G.cpu().numpy() - want).max() <= tol * np.abs(want).max()
+    assert np.abs(G.numpy() - want).max() <= tol * np.abs(want).max()
 
 
 def make_grams(rng, dims, R):
@@ -51,23 +46,23 @@ def make_grams(rng, dims, R):
 
 
 @pytest.mark.parametrize("rank", [8, 16, 32, 64, 20])
-def test_solve_cholesky(B, torch_cuda, rank):
+def test_solve_cholesky(B, gpu, rank):
     rng = np.random.default_rng(rank)
     dims = (300, 200, 100)
     _, grams = make_grams(rng, dims, rank)
     V = grams[0] * grams[2]
     M = rng.random((200, rank))
-    Gd = [dev_array(torch_cuda, g) for g in grams]
-    Md = dev_array(torch_cuda, M)
-    Ud = torch_cuda.zeros_like(Md)
+    Gd = [dev_array(B, g) for g in grams]
+    Md = dev_array(B, M)
+    Ud = B.DeviceArray(M.shape)
     fb = B.solve(Gd, 1, rank, Md, Ud, 0, 200)
     assert fb == 0
     want = np.linalg.solve(V, M.T).T
-    got = Ud.cpu().numpy()
+    got = Ud.numpy()
     assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want) * max(1, np.linalg.cond(V) * 1e-6)
 
 
-def test_solve_pinv_fallback(B, O, torch_cuda):
+def test_solve_pinv_fallback(B, O, gpu):
     """Two identical factor columns make V singular: the Cholesky fails and
     the minimum-norm solution M V^+ is returned."""
     rng = np.random.default_rng(4)
@@ -79,34 +74,35 @@ def test_solve_pinv_fallback(B, O, torch_cuda):
     grams = [u.T @ u for u in U]
     V = grams[1] * grams[2]
     M = rng.random((50, R))
-    Gd = [dev_array(torch_cuda, g) for g in grams]
-    Md = dev_array(torch_cuda, M)
-    Ud = torch_cuda.zeros_like(Md)
+    Gd = [dev_array(B, g) for g in grams]
+    Md = dev_array(B, M)
+    Ud = B.DeviceArray(M.shape)
     fb = B.solve(Gd, 0, R, Md, Ud, 0, 50)
     assert fb == 1
     P = O.pinv_sym(V)
     want = M @ P
-    got = Ud.cpu().numpy()
+    got = Ud.numpy()
     assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)
     assert np.allclose(P, np.linalg.pinv(V, hermitian=True), rtol=1e-6, atol=1e-9 * np.abs(P).max())
 
 
 @pytest.mark.parametrize("which", [0, 1])
 @pytest.mark.parametrize("dtype", [0, 1])
-def test_colnorm(B, O, torch_cuda, which, dtype):
+def test_colnorm(B, O, gpu, which, dtype):
     rng = np.random.default_rng(5)
     U = rng.random((3000, 32)) * 4 - 1
     U[:, 5] = 0.0  # zero column: lambda 0, column left at zero (L2)
-    Ud = dev_array(torch_cuda, U, dtype)
-    lam = torch_cuda.zeros(32, dtype=torch_cuda.float64, device="cuda")
+    Ud = dev_array(B, U, dtype)
+    held = Ud.numpy()  # the values as stored on the device, before the norm
+    lam = B.DeviceArray((32,))
     B.colnorm(Ud, dtype, 3000, 32, which, lam)
-    want, wl = O.kruskal_norm(Ud.double().cpu().numpy() if dtype else U, which)
+    want, wl = O.kruskal_norm(held, which)
     tol = 1e-12 if dtype == 0 else 1e-6
-    assert np.allclose(lam.cpu().numpy(), wl, rtol=tol)
-    assert np.allclose(Ud.double().cpu().numpy(), want, rtol=tol, atol=tol)
+    assert np.allclose(lam.numpy(), wl, rtol=tol)
+    assert np.allclose(Ud.numpy(), want, rtol=tol, atol=tol)
 
 
-def test_fit_matches_oracle(B, O, torch_cuda):
+def test_fit_matches_oracle(B, O, gpu):
     rng = np.random.default_rng(6)
     dims = (40, 30, 20)
     R = 16
@@ -116,25 +112,22 @@ def test_fit_matches_oracle(B, O, torch_cuda):
     M = rng.random((20, R))
     normX2 = 1234.5
     want = O.cpd_fit(U[2], M, lam, grams, normX2)
-    got = B.fit([dev_array(torch_cuda, g) for g in grams], R, dev_array(torch_cuda, lam),
-                dev_array(torch_cuda, U[2]), dev_array(torch_cuda, M), 0, 20, normX2)
+    got = B.fit([dev_array(B, g) for g in grams], R, dev_array(B, lam),
+                dev_array(B, U[2]), dev_array(B, M), 0, 20, normX2)
     assert abs(got - want) <= 1e-12 * max(1.0, abs(want))
 
 
-def test_hadamard_fill_vecsum(B, torch_cuda):
+def test_hadamard_fill_vecsum(B, gpu):
     rng = np.random.default_rng(7)
     x = rng.random(1000)
     y = rng.random(1000)
-    xd, yd = dev_array(torch_cuda, x), dev_array(torch_cuda, y)
+    xd, yd = dev_array(B, x), dev_array(B, y)
     B.hadamard(xd, yd, 1000)
-    torch_cuda.cuda.synchronize()
-    assert np.allclose(xd.cpu().numpy(), x * y, rtol=1e-15)
+    assert np.allclose(xd.numpy(), x * y, rtol=1e-15)
     B.value_fill(xd, 1000, 2.5)
-    torch_cuda.cuda.synchronize()
-    assert (xd.cpu().numpy() == 2.5).all()
+    assert (xd.numpy() == 2.5).all()
     B.vecsum(yd, 1000)
-    torch_cuda.cuda.synchronize()
-    assert abs(yd.cpu().numpy()[0] - y.sum()) <= 1e-12 * y.sum()
+    assert abs(yd.numpy()[0] - y.sum()) <= 1e-12 * y.sum()
 
 
 # ------------------------------------------------------------- CP-ALS
@@ -151,7 +144,7 @@ def small(B, O):
     return c1_like(B, O)
 
 
-def test_cpals_f64_trajectory_matches_oracle(B, O, torch_cuda, small):
+def test_cpals_f64_trajectory_matches_oracle(B, O, gpu, small):
     dims, t, h, at = small
     R = 16
     U0 = B.init_factors(1001, dims, R)
@@ -166,7 +159,7 @@ def test_cpals_f64_trajectory_matches_oracle(B, O, torch_cuda, small):
 
 
 @pytest.mark.parametrize("kernel", ["K_LVL1", "K_ATOMIC", "K_AUTO"])
-def test_cpals_kernels_agree(B, O, torch_cuda, small, kernel):
+def test_cpals_kernels_agree(B, O, gpu, small, kernel):
     dims, t, h, at = small
     R = 8
     U0 = B.init_factors(5, dims, R)
@@ -176,7 +169,7 @@ def test_cpals_kernels_agree(B, O, torch_cuda, small, kernel):
     assert np.abs(fits - fo).max() <= 1e-8
 
 
-def test_cpals_four_modes_and_short_modes(B, O, torch_cuda):
+def test_cpals_four_modes_and_short_modes(B, O, gpu):
     dims = (60, 700, 50, 90)
     t = B.Coo.generate(dims, 80000, gen_rank=6, seed=9)
     at = O.Alto(dims, t.cidx, t.vals, nprtn=8)
@@ -188,7 +181,7 @@ def test_cpals_four_modes_and_short_modes(B, O, torch_cuda):
 
 
 @pytest.mark.parametrize("precision,tol", [(1, 1e-4), (2, 5e-3)])
-def test_cpals_precision_forms(B, O, torch_cuda, small, precision, tol):
+def test_cpals_precision_forms(B, O, gpu, small, precision, tol):
     dims, t, h, at = small
     R = 16
     U0 = B.init_factors(1001, dims, R)
@@ -199,7 +192,7 @@ def test_cpals_precision_forms(B, O, torch_cuda, small, precision, tol):
     assert abs(fits[-1] - fo[-1]) <= tol
 
 
-def test_cpals_graph_replay_matches_eager(B, torch_cuda, small):
+def test_cpals_graph_replay_matches_eager(B, gpu, small):
     """A hipGraph-captured sweep replayed many times gives the eager fits."""
     dims, t, h, at = small
     R = 16
@@ -216,7 +209,7 @@ def test_cpals_graph_replay_matches_eager(B, torch_cuda, small):
         assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
 
 
-def test_cpals_tolerance_stops(B, O, torch_cuda, small):
+def test_cpals_tolerance_stops(B, O, gpu, small):
     dims, t, h, at = small
     U0 = B.init_factors(2, dims, 8)
     _, _, fo, _, _ = at.cpd(U0, 50, 1e-3)
@@ -225,7 +218,7 @@ def test_cpals_tolerance_stops(B, O, torch_cuda, small):
     assert len(fits) == len(fo) < 50
 
 
-def test_cpals_streamed_matches_resident(B, torch_cuda, small):
+def test_cpals_streamed_matches_resident(B, gpu, small):
     dims, t, h, at = small
     U0 = B.init_factors(4, dims, 16)
     a = B.DeviceTensor(h)
@@ -235,7 +228,7 @@ def test_cpals_streamed_matches_resident(B, torch_cuda, small):
     assert np.abs(fa - fb).max() <= 1e-10
 
 
-def test_cpals_mode_order_longest_first(B, O, torch_cuda, small):
+def test_cpals_mode_order_longest_first(B, O, gpu, small):
     dims, t, h, at = small
     U0 = B.init_factors(6, dims, 8)
     dev = B.DeviceTensor(h)

@@ -347,3 +347,17 @@ def test_cpd_singular_normal_matrix_uses_pinv(O):
     at = O.Alto(dims, c, v, nprtn=2)
     U, lam, fits, _, _ = at.cpd(U0, 3)
     assert np.isfinite(fits).all() and all(np.isfinite(u).all() for u in U)
+
+
+def test_alto_mttkrp_reused_at_growing_ranks(O):
+    """One oracle tensor serves several ranks in a row (its per-partition
+    buffers must grow with rows x rank, not rows alone)."""
+    rng = np.random.default_rng(24)
+    dims = (500, 400, 300)
+    c, v = random_coo(rng, dims, 30000)
+    at = O.Alto(dims, c, v, nprtn=8)
+    for R in (4, 16, 64, 128):
+        U = [rng.random((d, R)) for d in dims]
+        for m in range(3):
+            want = O.mttkrp_coo(c, v, U, m)
+            assert np.linalg.norm(at.mttkrp(U, m) - want) <= 1e-13 * np.linalg.norm(want)
