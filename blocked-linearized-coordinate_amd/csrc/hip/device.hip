@@ -34,6 +34,7 @@ constexpr int LVL3_MAX_MODE_LENGTH = 100;     // src/utils.hpp:13
 constexpr int DEFAULT_TPW = 4;                // tiles of 64 per wave
 constexpr int DEFAULT_NPRTN = 1;
 constexpr uint32_t LVL3_CHUNKS_PER_WG = 64;
+constexpr uint32_t HOT_EXCLUDE = 64;  // hot rows kept out of the row windows
 
 static int value_size(int vdt) { return vdt == BLCO_F64 ? 8 : 4; }
 
@@ -90,6 +91,108 @@ const ChunkTab* chunk_table(blco_dev* d, int tpw)
     return &d->tabs.back();
 }
 
+const WindowTab* window_table(blco_dev* d, int k)
+{
+    for (auto& t : d->wtabs)
+        if (t.k == k) return &t;
+    WindowTab t;
+    t.k = k;
+    const uint64_t nfine = d->fine_begin.size();
+    const int N = d->nmodes;
+    const uint32_t G = 1u << k;
+    t.first.push_back(0);
+    std::vector<uint64_t> lo, hi;  // per entry per mode
+    uint64_t f = 0;
+    for (uint64_t b = 0; b < d->nblocks; ++b) {
+        while (f < nfine && d->fine_block[f] == b) {
+            uint64_t g1 = f;
+            SuperChunk sc;
+            sc.begin = d->fine_begin[f];
+            sc.len = 0;
+            sc.block = (uint32_t)b;
+            uint64_t l[BLCO_MAX_MODES], h[BLCO_MAX_MODES];
+            for (int m = 0; m < N; ++m) { l[m] = ~0ull; h[m] = 0; }
+            while (g1 < nfine && d->fine_block[g1] == b && g1 - f < G) {
+                sc.len += d->fine_len[g1];
+                for (int m = 0; m < N; ++m) {
+                    l[m] = std::min(l[m], d->fine_lo[m * nfine + g1]);
+                    h[m] = std::max(h[m], d->fine_hi[m * nfine + g1]);
+                }
+                ++g1;
+            }
+            t.h_sc.push_back(sc);
+            for (int m = 0; m < N; ++m) { lo.push_back(l[m]); hi.push_back(h[m]); }
+            f = g1;
+        }
+        t.first.push_back((uint32_t)t.h_sc.size());
+    }
+    t.n = (uint32_t)t.h_sc.size();
+    t.h_range.resize((uint64_t)N * t.n * 2);
+    for (uint32_t e = 0; e < t.n; ++e)
+        for (int m = 0; m < N; ++m) {
+            t.h_range[((uint64_t)m * t.n + e) * 2] = lo[(uint64_t)e * N + m];
+            t.h_range[((uint64_t)m * t.n + e) * 2 + 1] = hi[(uint64_t)e * N + m];
+        }
+    if (t.n) {
+        if (hipMalloc(&t.d_sc, sizeof(SuperChunk) * t.n) != hipSuccess ||
+            hipMalloc(&t.d_range, sizeof(uint64_t) * t.h_range.size()) != hipSuccess) {
+            if (t.d_sc) (void)hipFree(t.d_sc);
+            fail(BLCO_ENOMEM, "device allocation of the window table failed");
+            return nullptr;
+        }
+        if (hipMemcpy(t.d_sc, t.h_sc.data(), sizeof(SuperChunk) * t.n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t.d_range, t.h_range.data(), sizeof(uint64_t) * t.h_range.size(),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(t.d_sc);
+            (void)hipFree(t.d_range);
+            fail(BLCO_EHIP, "upload of the window table failed");
+            return nullptr;
+        }
+        d->bytes += sizeof(SuperChunk) * t.n + sizeof(uint64_t) * t.h_range.size();
+    }
+    d->wtabs.push_back(t);
+    return &d->wtabs.back();
+}
+
+// Super-chunk size (2^k fine chunks) for a mode: minimises an estimate of
+// the HBM atomics (a fitting window flushes its rows once; a window that
+// does not fit flushes per nonzero), with enough workgroups to fill the GPU.
+int choose_window_k(blco_dev* d, int mode, int rank, int acc_bytes)
+{
+    const int cls = acc_bytes == 8 ? 0 : 1;
+    if (d->window_k[mode][cls]) return d->window_k[mode][cls] - 1;
+    const uint64_t nfine = d->fine_begin.size();
+    const uint64_t total = WINDOW_LDS_BYTES / ((uint64_t)rank * acc_bytes);
+    const uint64_t cap = total - std::min<uint64_t>(d->hot_list_h[mode].size(), total / 2);
+    int best = 0;
+    double best_cost = -1.0;
+    for (int k = 0; k <= 8; ++k) {
+        const uint64_t G = 1ull << k;
+        double cost = 0.0;
+        uint64_t entries = 0;
+        uint64_t f = 0;
+        while (f < nfine) {
+            const uint32_t b = d->fine_block[f];
+            uint64_t g1 = f, lo = ~0ull, hi = 0, len = 0;
+            while (g1 < nfine && d->fine_block[g1] == b && g1 - f < G) {
+                lo = std::min(lo, d->fine_lo[mode * nfine + g1]);
+                hi = std::max(hi, d->fine_hi[mode * nfine + g1]);
+                len += d->fine_len[g1];
+                ++g1;
+            }
+            const uint64_t rows = hi >= lo ? hi - lo + 1 : 0;
+            cost += rows <= cap ? (double)rows : (double)cap + 0.5 * (double)len * (rows - cap) / rows;
+            ++entries;
+            f = g1;
+        }
+        cost += 4.0 * (double)entries;  // fixed per-workgroup work
+        if (k > 0 && entries < 1024) break;  // keep >= 4 workgroups per CU
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = k; }
+    }
+    d->window_k[mode][cls] = best + 1;
+    return best;
+}
+
 static bool lvl3_fits(const blco_dev* d, int mode, int rank, int ftype)
 {
     uint64_t esz = ftype == BLCO_F64 ? 8 : 4;
@@ -101,7 +204,7 @@ int auto_kernel(const blco_dev* d, int mode, int rank, int ftype)
     // src/alto_dev.cu:1975,1996-2001: short modes -> lvl3, else lvl1.
     if (d->dims[mode] <= (uint64_t)LVL3_MAX_MODE_LENGTH && lvl3_fits(d, mode, rank, ftype))
         return BLCO_K_LVL3;
-    return BLCO_K_BATCHED;
+    return BLCO_K_BATCHED2;
 }
 
 static int ensure_partials(blco_dev* d, size_t bytes)
@@ -161,16 +264,16 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     int which;
     switch (kernel) {
         case BLCO_K_LVL1:
+        case BLCO_K_BATCHED: which = W_LVL1; break;
         case BLCO_K_LVL2:
-        case BLCO_K_BATCHED:
-        case BLCO_K_BATCHED2: which = W_LVL1; break;
+        case BLCO_K_BATCHED2: which = W_WINDOW; break;
         case BLCO_K_LVL3:
             which = W_LVL3;
             if (!lvl3_fits(d, mode, rank, ftype)) {
                 // the reference warns and proceeds (src/alto_dev.cu:1866); a
                 // privatised copy must fit in LDS here, so lvl1 runs instead
-                which = W_LVL1;
-                kernel = BLCO_K_BATCHED;
+                which = W_WINDOW;
+                kernel = BLCO_K_BATCHED2;
             }
             break;
         case BLCO_K_ATOMIC: which = W_ATOMIC; break;
@@ -207,11 +310,92 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     }
 
     int rc = BLCO_OK;
-    if (!d->stream_data) {
+    if (which == W_WINDOW) {
+        const int acc = ftype == BLCO_F64 ? 8 : 4;
+        const WindowTab* wt = window_table(d, choose_window_k(d, mode, rank, acc));
+        if (!wt) return BLCO_ENOMEM;
+        a.sc = wt->d_sc;
+        a.sc_range = wt->d_range + (uint64_t)mode * wt->n * 2;
+        const uint32_t total_rows = (uint32_t)(WINDOW_LDS_BYTES / ((uint64_t)rank * acc));
+        uint32_t hot_rows = (uint32_t)std::min<size_t>(d->hot_list_h[mode].size(), total_rows / 2);
+        if (hot_rows) {
+            const size_t need = (size_t)HOT_PARTS * hot_rows * rank * acc;
+            if (d->hotpart_bytes < need) {
+                if (d->hotpart) (void)hipFree(d->hotpart);
+                d->hotpart = nullptr;
+                d->hotpart_bytes = 0;
+                HIP_TRY(hipMalloc(&d->hotpart, need));
+                HIP_TRY(hipMemset(d->hotpart, 0, need));  // later calls leave it cleared
+                d->hotpart_bytes = need;
+            }
+        }
+        a.hot_rank = hot_rows ? d->d_hot_rank[mode] : nullptr;
+        a.hot_rows = hot_rows;
+        a.hotpart = d->hotpart;
+        a.hot_parts = HOT_PARTS;
+        a.lds_rows = total_rows - hot_rows;
+        const size_t lds = (size_t)total_rows * rank * acc;
+        const bool per_block = kernel == BLCO_K_LVL2;
+        const int vsz = value_size(d->vdtype);
+        if (!d->stream_data) {
+            a.coords = d->d_coords;
+            a.vals = d->d_vals;
+            a.coord_base = 0;
+            if (per_block) {
+                for (uint64_t b = 0; b < d->nblocks && !rc; ++b) {
+                    a.chunk0 = wt->first[b];
+                    a.nchunks = wt->first[b + 1] - wt->first[b];
+                    if (a.nchunks) rc = launch_mttkrp(W_WINDOW, a, ftype, a.nchunks, lds, st);
+                }
+            } else {
+                a.chunk0 = 0;
+                a.nchunks = wt->n;
+                if (wt->n) rc = launch_mttkrp(W_WINDOW, a, ftype, wt->n, lds, st);
+            }
+        } else {
+            // stream entry ranges of at most slot_nnz nonzeros within a block
+            size_t u = 0;
+            for (uint64_t b = 0; b < d->nblocks && !rc; ++b) {
+                uint32_t e = wt->first[b];
+                while (e < wt->first[b + 1] && !rc) {
+                    uint32_t e1 = e;
+                    uint64_t cnt = 0;
+                    while (e1 < wt->first[b + 1] && (e1 == e || cnt + wt->h_sc[e1].len <= d->slot_nnz)) {
+                        cnt += wt->h_sc[e1].len;
+                        ++e1;
+                    }
+                    if (cnt > d->slot_nnz)
+                        return fail(BLCO_EUNSUPPORTED, "a window of %llu nonzeros exceeds the %llu-nonzero slot",
+                                    (unsigned long long)cnt, (unsigned long long)d->slot_nnz);
+                    const int s = (int)(u & 1);
+                    const uint64_t begin = wt->h_sc[e].begin;
+                    HIP_TRY(hipStreamWaitEvent(d->copy_stream, d->ev_done[s], 0));
+                    HIP_TRY(hipMemcpyAsync(d->slot_coords[s], d->h_coords + begin, cnt * 8,
+                                           hipMemcpyHostToDevice, d->copy_stream));
+                    HIP_TRY(hipMemcpyAsync(d->slot_vals[s], (const char*)d->h_vals + begin * vsz,
+                                           cnt * vsz, hipMemcpyHostToDevice, d->copy_stream));
+                    HIP_TRY(hipEventRecord(d->ev_copied[s], d->copy_stream));
+                    HIP_TRY(hipStreamWaitEvent(st, d->ev_copied[s], 0));
+                    a.coords = d->slot_coords[s];
+                    a.vals = d->slot_vals[s];
+                    a.coord_base = begin;
+                    a.chunk0 = e;
+                    a.nchunks = e1 - e;
+                    rc = launch_mttkrp(W_WINDOW, a, ftype, a.nchunks, lds, st);
+                    HIP_TRY(hipEventRecord(d->ev_done[s], st));
+                    ++u;
+                    e = e1;
+                }
+            }
+        }
+        if (!rc && hot_rows)
+            rc = launch_hot_reduce(out, d->hotpart, d->d_hot_list[mode], hot_rows, rank, HOT_PARTS,
+                                   ftype == BLCO_F64 ? BLCO_F64 : BLCO_F32, st);
+    } else if (!d->stream_data) {
         a.coords = d->d_coords;
         a.vals = d->d_vals;
         a.coord_base = 0;
-        if (kernel == BLCO_K_LVL1 || kernel == BLCO_K_LVL2) {
+        if (kernel == BLCO_K_LVL1) {
             for (uint64_t b = 0; b < d->nblocks && !rc; ++b)
                 rc = launch_range(d, which, a, t, t->first[b], t->first[b + 1], ftype, st);
         } else {
@@ -337,20 +521,69 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
         if (masks[m] && (pos[m] < 0 || pos[m] > 63)) return fail(BLCO_EINVAL, "mode %d position %d", m, pos[m]);
     }
     // every decoded index must lie inside its mode: kernels gather factor
-    // rows and scatter output rows with these indices unchecked
-    for (uint64_t b = 0; b < nblocks; ++b) {
-        for (int m = 0; m < nmodes; ++m) {
+    // rows and scatter output rows with these indices unchecked.  The same
+    // pass records each fine chunk's per-mode row range.
+    std::vector<uint64_t> fbeg, flo, fhi;
+    std::vector<uint32_t> flen, fblk;
+    for (uint64_t b = 0; b < nblocks; ++b)
+        for (uint64_t s0 = 0; s0 < block_nnz[b]; s0 += FINE_CHUNK) {
+            fbeg.push_back(block_start[b] + s0);
+            flen.push_back((uint32_t)std::min<uint64_t>(FINE_CHUNK, block_nnz[b] - s0));
+            fblk.push_back((uint32_t)b);
+        }
+    const uint64_t nfine = fbeg.size();
+    // pass 1: bounds check and per-mode row counts
+    std::vector<std::vector<uint32_t>> hot(nmodes);
+    for (int m = 0; m < nmodes; ++m) {
+        std::vector<uint32_t> cnt;
+        const bool count = dims[m] <= (1ull << 31);
+        if (count) cnt.assign(dims[m], 0);
+        for (uint64_t b = 0; b < nblocks; ++b) {
             const uint64_t bc = block_coords[b * nmodes + m], mk = masks[m];
             const int ps = mk ? pos[m] : 0;
-            uint64_t mx = 0;
             for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
-                uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                const uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                if (v >= dims[m])
+                    return fail(BLCO_EINVAL, "block %llu decodes mode-%d index %llu >= %llu",
+                                (unsigned long long)b, m, (unsigned long long)v,
+                                (unsigned long long)dims[m]);
+                if (count && cnt[v] != 0xFFFFFFFFu) ++cnt[v];
+            }
+        }
+        if (count && nnz) {
+            const uint64_t thr = std::max<uint64_t>(2, nnz / 2048);
+            std::vector<uint32_t> cand;
+            for (uint64_t r = 0; r < dims[m]; ++r)
+                if (cnt[r] >= thr) cand.push_back((uint32_t)r);
+            std::sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
+                return cnt[x] != cnt[y] ? cnt[x] > cnt[y] : x < y;
+            });
+            if (cand.size() > HOT_EXCLUDE) cand.resize(HOT_EXCLUDE);
+            hot[m] = cand;
+        }
+    }
+    // pass 2: per fine chunk, per mode, the row range of the non-hot rows
+    flo.assign((uint64_t)nmodes * nfine, ~0ull);
+    fhi.assign((uint64_t)nmodes * nfine, 0);
+    for (int m = 0; m < nmodes; ++m) {
+        std::vector<uint8_t> is_hot;
+        if (!hot[m].empty()) {
+            is_hot.assign(dims[m], 0);
+            for (uint32_t r : hot[m]) is_hot[r] = 1;
+        }
+        const uint64_t mk = masks[m];
+        const int ps = mk ? pos[m] : 0;
+        for (uint64_t f = 0; f < nfine; ++f) {
+            const uint64_t bc = block_coords[(uint64_t)fblk[f] * nmodes + m];
+            uint64_t mn = ~0ull, mx = 0;
+            for (uint64_t i = fbeg[f]; i < fbeg[f] + flen[f]; ++i) {
+                const uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                if (!is_hot.empty() && is_hot[v]) continue;
+                mn = v < mn ? v : mn;
                 mx = v > mx ? v : mx;
             }
-            if (block_nnz[b] && mx >= dims[m])
-                return fail(BLCO_EINVAL, "block %llu decodes mode-%d index %llu >= %llu",
-                            (unsigned long long)b, m, (unsigned long long)mx,
-                            (unsigned long long)dims[m]);
+            flo[m * nfine + f] = mn;
+            fhi[m * nfine + f] = mx;
         }
     }
     if (!have_kfd()) return fail(BLCO_ENODEV, "no GPU (/dev/kfd absent)");
@@ -373,10 +606,29 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
     d->nblocks = nblocks;
     d->bstart.assign(block_start, block_start + nblocks);
     d->bnnz.assign(block_nnz, block_nnz + nblocks);
+    d->fine_begin.swap(fbeg);
+    d->fine_len.swap(flen);
+    d->fine_block.swap(fblk);
+    d->fine_lo.swap(flo);
+    d->fine_hi.swap(fhi);
+    for (int m = 0; m < nmodes; ++m) d->hot_list_h[m] = hot[m];
     auto cleanup = [&](int rc) { blco_dev_free(d); return rc; };
     if (hipGetDevice(&d->device) != hipSuccess) return cleanup(fail(BLCO_EHIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(BLCO_EHIP, "stream creation failed"));
+    for (int m = 0; m < nmodes; ++m) {
+        const std::vector<uint32_t>& hl = d->hot_list_h[m];
+        if (hl.empty()) continue;
+        std::vector<int32_t> rk(dims[m], -1);
+        for (size_t h = 0; h < hl.size(); ++h) rk[hl[h]] = (int32_t)h;
+        if (hipMalloc(&d->d_hot_list[m], sizeof(uint32_t) * hl.size()) != hipSuccess ||
+            hipMalloc(&d->d_hot_rank[m], sizeof(int32_t) * dims[m]) != hipSuccess)
+            return cleanup(fail(BLCO_ENOMEM, "device allocation failed (hot rows)"));
+        if (hipMemcpy(d->d_hot_list[m], hl.data(), sizeof(uint32_t) * hl.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d->d_hot_rank[m], rk.data(), sizeof(int32_t) * dims[m], hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(BLCO_EHIP, "upload of the hot-row tables failed"));
+        d->bytes += sizeof(uint32_t) * hl.size() + sizeof(int32_t) * dims[m];
+    }
     if (nblocks) {
         if (hipMalloc(&d->d_bcoords, sizeof(uint64_t) * nblocks * nmodes) != hipSuccess)
             return cleanup(fail(BLCO_ENOMEM, "device allocation failed (block coordinates)"));
@@ -455,6 +707,15 @@ void blco_dev_free(blco_dev* d)
     if (d->copy_stream) (void)hipStreamSynchronize(d->copy_stream);
     for (auto& t : d->tabs)
         if (t.d) (void)hipFree(t.d);
+    for (auto& t : d->wtabs) {
+        if (t.d_sc) (void)hipFree(t.d_sc);
+        if (t.d_range) (void)hipFree(t.d_range);
+    }
+    for (int m = 0; m < BLCO_MAX_MODES; ++m) {
+        if (d->d_hot_list[m]) (void)hipFree(d->d_hot_list[m]);
+        if (d->d_hot_rank[m]) (void)hipFree(d->d_hot_rank[m]);
+    }
+    if (d->hotpart) (void)hipFree(d->hotpart);
     if (d->d_bcoords) (void)hipFree(d->d_bcoords);
     if (d->d_coords) (void)hipFree(d->d_coords);
     if (d->d_vals) (void)hipFree(d->d_vals);

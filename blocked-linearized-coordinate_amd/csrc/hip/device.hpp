@@ -19,6 +19,18 @@ struct StreamUnit {
     uint64_t nnz;
 };
 
+// Super-chunks of 2^k fine chunks (within one block) with per-mode row
+// ranges, for the row-window kernel.
+struct WindowTab {
+    int k = 0;
+    SuperChunk* d_sc = nullptr;
+    uint64_t* d_range = nullptr;  // [mode][entry][2]
+    uint32_t n = 0;
+    std::vector<uint32_t> first;  // first entry of each block (nblocks + 1)
+    std::vector<uint64_t> h_range;
+    std::vector<SuperChunk> h_sc;
+};
+
 struct ChunkTab {
     int tpw = 0;
     Chunk* d = nullptr;
@@ -44,6 +56,20 @@ struct blco_dev {
     uint64_t* d_coords = nullptr;
     void* d_vals = nullptr;
     std::list<blco::ChunkTab> tabs;
+    std::list<blco::WindowTab> wtabs;
+    // fine chunks (FINE_CHUNK nonzeros, aligned per block) and their
+    // per-mode output row ranges: fine_lo/hi[mode * nfine + f]
+    std::vector<uint64_t> fine_begin;
+    std::vector<uint32_t> fine_len, fine_block;
+    std::vector<uint64_t> fine_lo, fine_hi;
+    int window_k[BLCO_MAX_MODES][3] = {};  // chosen k + 1 per (mode, accumulator size class)
+    // most frequent rows per mode (count >= nnz / 2048, at most HOT_EXCLUDE),
+    // in decreasing frequency; fine ranges exclude them
+    std::vector<uint32_t> hot_list_h[BLCO_MAX_MODES];
+    uint32_t* d_hot_list[BLCO_MAX_MODES] = {};
+    int32_t* d_hot_rank[BLCO_MAX_MODES] = {};
+    void* hotpart = nullptr;
+    size_t hotpart_bytes = 0;
     // streaming
     uint64_t* h_coords = nullptr;
     void* h_vals = nullptr;
@@ -63,6 +89,8 @@ struct blco_dev {
 namespace blco {
 
 const ChunkTab* chunk_table(blco_dev* d, int tpw);
+const WindowTab* window_table(blco_dev* d, int k);
+int choose_window_k(blco_dev* d, int mode, int rank, int acc_bytes);
 int auto_kernel(const blco_dev* d, int mode, int rank, int ftype);
 int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
            int nprtn, int tpw, hipStream_t st, int* used_kernel);

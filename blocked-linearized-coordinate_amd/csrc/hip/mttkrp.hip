@@ -267,6 +267,115 @@ __global__ __launch_bounds__(WG) void k_mttkrp_lvl3(MttkrpArgs a)
     }
 }
 
+// --------------------------------------------------------- row window
+
+// One workgroup per super-chunk.  LDS holds two accumulators: the hot area
+// (the output mode's most frequent rows, by frequency rank) and the row
+// window [lo, lo + lds_rows) of the super-chunk's other rows.  A segment goes
+// to the hot area, else to the window, else straight to HBM.  At the end the
+// window is added into the output and the hot area into partial copy
+// (workgroup % hot_parts), both with coalesced HBM atomics skipping zero
+// cells; k_hot_reduce folds the copies into the output.
+template <int N, typename FT, typename VT, int C, int L, bool EXACT>
+__global__ __launch_bounds__(WG) void k_mttkrp_window(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    __shared__ uint64_t s_idx[WAVES_PER_WG][N][WAVE + WAVE / L];
+    __shared__ AT s_val[WAVES_PER_WG][WAVE + WAVE / L];
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const int R = a.rank;
+    AT* s_hot = reinterpret_cast<AT*>(s_dyn);
+    AT* s_win = s_hot + (uint64_t)a.hot_rows * R;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t e = a.chunk0 + g;  // grid == entry count
+    const SuperChunk sc = a.sc[e];
+    const uint64_t lo = a.sc_range[2 * (uint64_t)e], hi = a.sc_range[2 * (uint64_t)e + 1];
+    const uint64_t wrows = hi >= lo ? min(hi - lo + 1, (uint64_t)a.lds_rows) : 0;
+    const uint64_t cells = ((uint64_t)a.hot_rows + wrows) * R;
+    for (uint64_t i = threadIdx.x; i < cells; i += WG) s_hot[i] = (AT)0;
+    __syncthreads();
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)sc.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const uint32_t ntiles = (sc.len + WAVE - 1) / WAVE;
+    for (uint32_t t = w; t < ntiles; t += WAVES_PER_WG) {
+        const uint32_t t0 = t * WAVE;
+        const uint32_t nvalid = min((uint32_t)WAVE, sc.len - t0);
+        stage_tile<N, L, VT, AT>(a, bc, sc.begin + t0, nvalid, lane, s_idx[w], s_val[w]);
+        for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+            const int col = cb + sub * C;
+            segment_tile<N, FT, C, L, EXACT>(a, nvalid, slot, col, s_idx[w], s_val[w],
+                [&](uint64_t row, const AT (&acc)[C]) {
+                    const int32_t hr = a.hot_rank ? a.hot_rank[row] : -1;
+                    AT* o;
+                    bool lds = true;
+                    if (hr >= 0 && (uint32_t)hr < a.hot_rows) o = s_hot + (uint64_t)hr * R + col;
+                    else if (row >= lo && row - lo < wrows) o = s_win + (row - lo) * (uint64_t)R + col;
+                    else { o = out + row * (uint64_t)R + col; lds = false; }
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) {
+                        if (!(EXACT || col + cc < R)) continue;
+                        if (lds) atomic_add_lds(o + cc, acc[cc]);
+                        else atomic_add_global(o + cc, acc[cc]);
+                    }
+                });
+        }
+        wave_lds_sync();
+    }
+    __syncthreads();
+    const uint64_t hcells = (uint64_t)a.hot_rows * R;
+    if (hcells) {
+        AT* hp = reinterpret_cast<AT*>(a.hotpart) + (uint64_t)(g % (uint32_t)a.hot_parts) * hcells;
+        for (uint64_t i = threadIdx.x; i < hcells; i += WG) {
+            const AT v = s_hot[i];
+            if (v != (AT)0) atomic_add_global(hp + i, v);
+        }
+    }
+    AT* dst = out + lo * (uint64_t)R;
+    for (uint64_t i = threadIdx.x; i < wrows * R; i += WG) {
+        const AT v = s_win[i];
+        if (v != (AT)0) atomic_add_global(dst + i, v);
+    }
+}
+
+// out[hot_list[h]][c] += sum_p part[p][h][c]; the copies are cleared for the
+// next call.
+template <typename AT>
+__global__ __launch_bounds__(WG) void k_hot_reduce(AT* __restrict__ out, AT* __restrict__ part,
+                                                   const uint32_t* __restrict__ hot_list,
+                                                   uint32_t hot_rows, int R, int parts)
+{
+    const uint64_t cells = (uint64_t)hot_rows * R;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < cells; i += (uint64_t)gridDim.x * WG) {
+        AT s = 0;
+        for (int p = 0; p < parts; ++p) {
+            s += part[(uint64_t)p * cells + i];
+            part[(uint64_t)p * cells + i] = 0;
+        }
+        const uint64_t h = i / R, c = i % R;
+        out[(uint64_t)hot_list[h] * R + c] += s;
+    }
+}
+
+int launch_hot_reduce(void* out, void* hotpart, const uint32_t* hot_list, uint32_t hot_rows,
+                      int rank, int parts, int otype, hipStream_t st)
+{
+    const uint64_t cells = (uint64_t)hot_rows * rank;
+    if (!cells) return BLCO_OK;
+    uint32_t grid = (uint32_t)std::min<uint64_t>((cells + WG - 1) / WG, 1024);
+    if (otype == BLCO_F64)
+        hipLaunchKernelGGL(k_hot_reduce<double>, dim3(grid), dim3(WG), 0, st, (double*)out,
+                           (double*)hotpart, hot_list, hot_rows, rank, parts);
+    else
+        hipLaunchKernelGGL(k_hot_reduce<float>, dim3(grid), dim3(WG), 0, st, (float*)out,
+                           (float*)hotpart, hot_list, hot_rows, rank, parts);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
 // ------------------------------------------------------------- atomic
 
 template <int N, typename FT, typename VT, int C, int L, bool EXACT>
@@ -382,6 +491,7 @@ static const void* kernel_ptr(int which)
 {
     if (which == W_LVL1) return (const void*)&k_mttkrp_lvl1<N, FT, VT, C, L, EX>;
     if (which == W_LVL3) return (const void*)&k_mttkrp_lvl3<N, FT, VT, C, L, EX>;
+    if (which == W_WINDOW) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX>;
     return (const void*)&k_mttkrp_atomic<N, FT, VT, C, L, EX>;
 }
 
