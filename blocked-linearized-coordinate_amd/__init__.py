@@ -22,12 +22,14 @@ LIB_PATH = os.path.join(HERE, "lib", "libblco.so")
 F64, F32, BF16 = 0, 1, 2
 K_LVL1, K_LVL2, K_LVL3, K_AUTO, K_BATCHED, K_BATCHED2, K_ATOMIC, K_SELECT = 1, 2, 3, 10, 11, 12, 13, 20
 MAX_MODES = 5
+SEL_NFEAT, SEL_NCAND = 24, 10
 
 _lib = None
 
 u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 vp = C.c_void_p
 
 
@@ -116,6 +118,27 @@ def lib():
             "blco_cpals_free": (None, [vp]),
             "blco_cpals": (C.c_int, [vp, C.POINTER(CpalsOpts), C.POINTER(vp), f64p, f64p,
                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+            "blco_mttkrp_time": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), vp, C.c_int,
+                                           C.c_int, C.c_int, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_int)]),
+            "blco_mlp_create": (C.c_int, [C.c_int, i32p, C.c_uint64, C.POINTER(vp)]),
+            "blco_mlp_free": (None, [vp]),
+            "blco_mlp_nparams": (C.c_uint64, [vp]),
+            "blco_mlp_get_params": (C.c_int, [vp, vp, vp]),
+            "blco_mlp_set_params": (C.c_int, [vp, f32p]),
+            "blco_mlp_forward": (C.c_int, [vp, f32p, C.c_int, f32p]),
+            "blco_mlp_loss_grad": (C.c_int, [vp, f32p, f32p, f32p, C.c_int, C.POINTER(C.c_float)]),
+            "blco_mlp_adam_step": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+            "blco_mlp_train": (C.c_int, [vp, f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float,
+                                         C.c_uint64, vp]),
+            "blco_sel_candidate": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "blco_sel_valid": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "blco_mode_features": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, f32p]),
+            "blco_selector_install": (C.c_int, [C.c_int, i32p, f32p, f32p, f32p]),
+            "blco_selector_load": (C.c_int, [C.c_char_p]),
+            "blco_selector_installed": (C.c_int, []),
+            "blco_select_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -398,6 +421,21 @@ class DeviceTensor:
     def bytes(self):
         return lib().blco_dev_bytes(self.h)
 
+    def features(self, mode, rank, factor_dtype=F32):
+        """Selector features of (mode, rank, factor type), SEL_NFEAT floats."""
+        f = np.zeros(SEL_NFEAT, np.float32)
+        _check(lib().blco_mode_features(self.h, mode, rank, factor_dtype, f))
+        return f
+
+    def candidate_valid(self, mode, rank, factor_dtype, c):
+        return bool(lib().blco_sel_valid(self.h, mode, rank, factor_dtype, c))
+
+    def select_kernel(self, mode, rank, factor_dtype=F32):
+        """(kernel id, lvl3 partitions) that kernel id 20 runs for this mode."""
+        k, n = C.c_int(0), C.c_int(0)
+        _check(lib().blco_select_kernel(self.h, mode, rank, factor_dtype, C.byref(k), C.byref(n)))
+        return k.value, n.value
+
     def auto_kernel(self, mode, rank):
         return lib().blco_auto_kernel(self.h, mode, rank)
 
@@ -416,6 +454,117 @@ def mttkrp(dev, mode, rank, factors, out, factor_dtype=F64, kernel=K_AUTO, nprtn
     ptrs = _ptr_array([_dptr(f) if f is not None else 0 for f in factors])
     _check(lib().blco_mttkrp(dev.h, mode, rank, factor_dtype, ptrs, _dptr(out), kernel, nprtn,
                              thread_cf, _stream(stream)))
+
+
+def mttkrp_time(dev, mode, rank, factors, out, factor_dtype=F64, kernel=K_AUTO, nprtn=0, reps=3):
+    """Mean event-timed ms of one MTTKRP launch sequence (after one untimed
+    call), and the kernel id that ran."""
+    ptrs = _ptr_array([_dptr(f) for f in factors])
+    ms, used = C.c_float(0), C.c_int(0)
+    _check(lib().blco_mttkrp_time(dev.h, mode, rank, factor_dtype, ptrs, _dptr(out), kernel, nprtn,
+                                  reps, C.byref(ms), C.byref(used)))
+    return ms.value, used.value
+
+
+def sel_candidate(c):
+    """Candidate c of the selector -> (kernel id, lvl3 partitions)."""
+    k, n = C.c_int(0), C.c_int(0)
+    _check(lib().blco_sel_candidate(c, C.byref(k), C.byref(n)))
+    return k.value, n.value
+
+
+def selector_install(widths, params, xmean, xstd):
+    """Installs a selector network (widths[0] = SEL_NFEAT, widths[-1] =
+    SEL_NCAND); widths=None uninstalls."""
+    if widths is None:
+        _check(lib().blco_selector_install(0, np.zeros(1, np.int32), np.zeros(1, np.float32),
+                                           np.zeros(1, np.float32), np.zeros(1, np.float32)))
+        return
+    w = np.ascontiguousarray(widths, np.int32)
+    _check(lib().blco_selector_install(len(w) - 1, w, np.ascontiguousarray(params, np.float32),
+                                       np.ascontiguousarray(xmean, np.float32),
+                                       np.ascontiguousarray(xstd, np.float32)))
+
+
+def selector_load(path):
+    _check(lib().blco_selector_load(os.fsencode(path)))
+
+
+def selector_installed():
+    return bool(lib().blco_selector_installed())
+
+
+def write_selector(path, widths, params, xmean, xstd):
+    """Writes a selector file: b"BLCOSEL1", int32 L, int32 widths[L+1],
+    f32 mean[NFEAT], f32 scale[NFEAT], f32 parameters."""
+    w = np.asarray(widths, np.int32)
+    with open(path, "wb") as f:
+        f.write(b"BLCOSEL1")
+        f.write(np.int32(len(w) - 1).tobytes())
+        f.write(w.tobytes())
+        f.write(np.asarray(xmean, np.float32).tobytes())
+        f.write(np.asarray(xstd, np.float32).tobytes())
+        f.write(np.asarray(params, np.float32).tobytes())
+
+
+class Mlp:
+    """Dense ReLU network run and trained by the library's HIP kernels."""
+
+    def __init__(self, widths, seed=0):
+        self.widths = [int(w) for w in widths]
+        h = vp()
+        _check(lib().blco_mlp_create(len(self.widths) - 1, np.asarray(self.widths, np.int32),
+                                     seed, C.byref(h)))
+        self.h = h.value
+        self.nparams = int(lib().blco_mlp_nparams(self.h))
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        _check(lib().blco_mlp_get_params(self.h, p.ctypes.data, None))
+        return p
+
+    def grads(self):
+        g = np.zeros(self.nparams, np.float32)
+        _check(lib().blco_mlp_get_params(self.h, None, g.ctypes.data))
+        return g
+
+    def set_params(self, p):
+        _check(lib().blco_mlp_set_params(self.h, np.ascontiguousarray(p, np.float32)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], self.widths[-1]), np.float32)
+        _check(lib().blco_mlp_forward(self.h, X, X.shape[0], Y))
+        return Y
+
+    def loss_grad(self, X, T, M):
+        l = C.c_float(0)
+        _check(lib().blco_mlp_loss_grad(self.h, np.ascontiguousarray(X, np.float32),
+                                        np.ascontiguousarray(T, np.float32),
+                                        np.ascontiguousarray(M, np.float32), len(X), C.byref(l)))
+        return l.value
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(lib().blco_mlp_adam_step(self.h, lr, beta1, beta2, eps))
+
+    def train(self, X, T, M, epochs, batch, lr, seed=0):
+        trace = np.zeros(max(epochs, 1), np.float32)
+        _check(lib().blco_mlp_train(self.h, np.ascontiguousarray(X, np.float32),
+                                    np.ascontiguousarray(T, np.float32),
+                                    np.ascontiguousarray(M, np.float32), len(X), epochs, batch, lr,
+                                    seed, trace.ctypes.data))
+        return trace[:epochs]
+
+    def free(self):
+        if self.h:
+            lib().blco_mlp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def gram(U, dtype, rows, rank, G, stream=None):

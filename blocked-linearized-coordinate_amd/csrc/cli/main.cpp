@@ -224,7 +224,7 @@ int main(int argc, char** argv)
     if (hipSetDevice(device) != hipSuccess) { fprintf(stderr, "hipSetDevice failed\n"); return -1; }
     if (do_check) max_iters = 1;
     if (!stream_data) max_block_size = X->nnz;
-    if (num_partitions == 0) num_partitions = 1;
+    if (num_partitions == 0) num_partitions = omp_get_max_threads();  // src/main.cpp:125
     cpals_set_precision(precision);
     cpals_set_mode_order(mode_order);
     cpals_set_graph(graph);

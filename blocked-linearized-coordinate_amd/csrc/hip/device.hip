@@ -154,46 +154,183 @@ const WindowTab* window_table(blco_dev* d, int k)
     return &d->wtabs.back();
 }
 
-// Super-chunk size (2^k fine chunks) for a mode: minimises an estimate of
-// the HBM atomics (a fitting window flushes its rows once; a window that
-// does not fit flushes per nonzero), with enough workgroups to fill the GPU.
-int choose_window_k(blco_dev* d, int mode, int rank, int acc_bytes)
+// Partial rows of one window plan stay under this many bytes; past it the
+// cost model moves to larger super-chunks.
+constexpr uint64_t PART_MAX_BYTES = 4ull << 30;
+// An HBM atomic add of one row costs about this many plain row stores.
+constexpr double ATOMIC_ROW_COST = 32.0;
+
+// Per-entry use of the LDS window: it pays when storing the window's rows
+// (and reading them back in the reduce) costs less than the HBM atomics of
+// the entry's nonzeros that it absorbs.
+static inline bool use_window(uint64_t rows, uint64_t cap, double len)
 {
-    const int cls = acc_bytes == 8 ? 0 : 1;
-    if (d->window_k[mode][cls]) return d->window_k[mode][cls] - 1;
-    const uint64_t nfine = d->fine_begin.size();
-    const uint64_t total = WINDOW_LDS_BYTES / ((uint64_t)rank * acc_bytes);
-    const uint64_t cap = total - std::min<uint64_t>(d->hot_list_h[mode].size(), total / 2);
-    int best = 0;
-    double best_cost = -1.0;
-    for (int k = 0; k <= 8; ++k) {
-        const uint64_t G = 1ull << k;
-        double cost = 0.0;
-        uint64_t entries = 0;
-        uint64_t f = 0;
-        while (f < nfine) {
-            const uint32_t b = d->fine_block[f];
-            uint64_t g1 = f, lo = ~0ull, hi = 0, len = 0;
-            while (g1 < nfine && d->fine_block[g1] == b && g1 - f < G) {
-                lo = std::min(lo, d->fine_lo[mode * nfine + g1]);
-                hi = std::max(hi, d->fine_hi[mode * nfine + g1]);
-                len += d->fine_len[g1];
-                ++g1;
-            }
-            const uint64_t rows = hi >= lo ? hi - lo + 1 : 0;
-            cost += rows <= cap ? (double)rows : (double)cap + 0.5 * (double)len * (rows - cap) / rows;
-            ++entries;
-            f = g1;
-        }
-        cost += 4.0 * (double)entries;  // fixed per-workgroup work
-        if (k > 0 && entries < 1024) break;  // keep >= 4 workgroups per CU
-        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = k; }
-    }
-    d->window_k[mode][cls] = best + 1;
-    return best;
+    const uint64_t w = std::min(rows, cap);
+    const double absorbed = rows ? len * (double)w / (double)rows : 0.0;
+    return w > 0 && 2.0 * (double)w < ATOMIC_ROW_COST * absorbed;
 }
 
-static bool lvl3_fits(const blco_dev* d, int mode, int rank, int ftype)
+// Window rows of each super-chunk of 2^k fine chunks for a mode.
+template <typename F>
+static void walk_entries(const blco_dev* d, int mode, int k, F&& visit)
+{
+    const uint64_t nfine = d->fine_begin.size();
+    const uint64_t G = 1ull << k;
+    uint64_t f = 0;
+    while (f < nfine) {
+        const uint32_t b = d->fine_block[f];
+        uint64_t g1 = f, lo = ~0ull, hi = 0, len = 0;
+        while (g1 < nfine && d->fine_block[g1] == b && g1 - f < G) {
+            lo = std::min(lo, d->fine_lo[mode * nfine + g1]);
+            hi = std::max(hi, d->fine_hi[mode * nfine + g1]);
+            len += d->fine_len[g1];
+            ++g1;
+        }
+        visit(hi >= lo ? hi - lo + 1 : 0, len);
+        f = g1;
+    }
+}
+
+// Super-chunk size (2^k fine chunks) and hot-area size for a mode: minimise
+// an estimate of the HBM work -- each used window row is stored once and
+// read once by the reduce, each hot area likewise; nonzeros outside a window
+// (or in hot rows when there is no hot area) are added into the output one
+// by one -- keeping >= 1024 workgroups and the partial rows under
+// PART_MAX_BYTES.
+static void choose_window(const blco_dev* d, int mode, uint32_t total, uint64_t row_bytes, int* k_out,
+                          uint32_t* hot_out)
+{
+    const uint32_t H = (uint32_t)std::min<size_t>(d->hot_list_h[mode].size(), total / 2);
+    uint64_t hot_nnz = 0;
+    for (uint32_t h = 0; h < H; ++h) hot_nnz += d->hot_cnt_h[mode][h];
+    const double hf = d->nnz ? (double)hot_nnz / (double)d->nnz : 0.0;
+    int best_k = 0, last_k = 0;
+    uint32_t best_h = 0;
+    double best_cost = -1.0;
+    for (int k = 0; k <= 8; ++k) {
+        for (uint32_t hot : {H, 0u}) {
+            const uint64_t cap = total - hot;
+            double cost = 0.0;
+            uint64_t entries = 0, prows = 0;
+            walk_entries(d, mode, k, [&](uint64_t rows, uint64_t len) {
+                const double l = (double)len * (1.0 - hf);  // nonzeros outside hot rows
+                if (use_window(rows, cap, l)) {
+                    const uint64_t w = std::min(rows, cap);
+                    prows += w;
+                    cost += 2.0 * (double)w + ATOMIC_ROW_COST * l * (double)(rows - w) / (double)rows;
+                } else {
+                    cost += ATOMIC_ROW_COST * l;
+                }
+                ++entries;
+            });
+            if (k > 0 && entries < 1024) goto done;  // keep >= 4 workgroups per CU
+            prows += (uint64_t)hot * entries;
+            cost += hot ? 2.0 * hot * (double)entries : 4.0 * ATOMIC_ROW_COST * (double)hot_nnz;
+            cost += 4.0 * (double)entries;  // fixed per-workgroup work
+            last_k = k;
+            if (prows * row_bytes > PART_MAX_BYTES) continue;
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                best_k = k;
+                best_h = hot;
+            }
+            if (H == 0) break;
+        }
+    }
+done:
+    if (best_cost < 0) {  // nothing under the cap: largest super-chunks, no hot area
+        best_k = last_k;
+        best_h = 0;
+    }
+    *k_out = best_k;
+    *hot_out = best_h;
+}
+
+const WindowPlan* window_plan(blco_dev* d, int mode, int rank, int acc)
+{
+    for (auto& p : d->wplans)
+        if (p.mode == mode && p.rank == rank && p.acc == acc) return &p;
+    WindowPlan p;
+    p.mode = mode;
+    p.rank = rank;
+    p.acc = acc;
+    const uint64_t row_bytes = (uint64_t)rank * acc;
+    const uint32_t total = (uint32_t)(WINDOW_LDS_BYTES / row_bytes);
+    int k = 0;
+    choose_window(d, mode, total, row_bytes, &k, &p.hot_rows);
+    p.lds_rows = total - p.hot_rows;
+    p.wt = window_table(d, k);
+    if (!p.wt) return nullptr;
+    const WindowTab& t = *p.wt;
+    const uint64_t* rg = t.h_range.data() + (uint64_t)mode * t.n * 2;
+    uint64_t hot_nnz = 0;
+    for (uint32_t h = 0; h < p.hot_rows; ++h) hot_nnz += d->hot_cnt_h[mode][h];
+    const double hf = d->nnz ? (double)hot_nnz / (double)d->nnz : 0.0;
+    // partial rows: each entry's window (when it pays), then each hot area;
+    // poff[e + 1] - poff[e] is entry e's window height
+    std::vector<uint64_t> poff(t.n + 1);
+    uint64_t P = 0;
+    for (uint32_t e = 0; e < t.n; ++e) {
+        poff[e] = P;
+        const uint64_t lo = rg[2 * e], hi = rg[2 * e + 1];
+        const uint64_t rows = hi >= lo ? hi - lo + 1 : 0;
+        if (use_window(rows, p.lds_rows, (double)t.h_sc[e].len * (1.0 - hf)))
+            P += std::min<uint64_t>(rows, p.lds_rows);
+    }
+    poff[t.n] = P;
+    p.hot_poff = P;
+    P += (uint64_t)t.n * p.hot_rows;
+    p.prows = P;
+    if (P > 0xFFFFFFFFull) {
+        fail(BLCO_EUNSUPPORTED, "%llu window partial rows exceed the 32-bit reduce index", (unsigned long long)P);
+        return nullptr;
+    }
+    // output row of every partial row, then the partial rows grouped by it
+    std::vector<uint64_t> prow(P);
+    for (uint32_t e = 0; e < t.n; ++e) {
+        const uint64_t lo = rg[2 * e];
+        for (uint64_t i = 0; i < poff[e + 1] - poff[e]; ++i) prow[poff[e] + i] = lo + i;
+        for (uint32_t h = 0; h < p.hot_rows; ++h)
+            prow[p.hot_poff + (uint64_t)e * p.hot_rows + h] = d->hot_list_h[mode][h];
+    }
+    std::vector<uint32_t> order(P);
+    if (d->dims[mode] <= (1ull << 27)) {  // counting sort by output row
+        std::vector<uint32_t> at(d->dims[mode] + 1, 0);
+        for (uint64_t i = 0; i < P; ++i) ++at[prow[i] + 1];
+        for (uint64_t r = 0; r < d->dims[mode]; ++r) at[r + 1] += at[r];
+        for (uint64_t i = 0; i < P; ++i) order[at[prow[i]]++] = (uint32_t)i;
+    } else {
+        for (uint64_t i = 0; i < P; ++i) order[i] = (uint32_t)i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return prow[x] < prow[y]; });
+    }
+    std::vector<uint64_t> rows, ptr;
+    for (uint64_t i = 0; i < P; ++i)
+        if (i == 0 || prow[order[i]] != prow[order[i - 1]]) {
+            rows.push_back(prow[order[i]]);
+            ptr.push_back(i);
+        }
+    ptr.push_back(P);
+    p.nred = rows.size();
+    auto up = [&](auto** dst, const auto& v) {
+        const size_t bytes = sizeof(v[0]) * v.size();
+        if (!bytes) return true;
+        if (hipMalloc(dst, bytes) != hipSuccess) return false;
+        d->bytes += bytes;
+        return hipMemcpy(*dst, v.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(&p.d_poff, poff) || !up(&p.d_rows, rows) || !up(&p.d_ptr, ptr) || !up(&p.d_idx, order)) {
+        if (p.d_poff) (void)hipFree(p.d_poff);
+        if (p.d_rows) (void)hipFree(p.d_rows);
+        if (p.d_ptr) (void)hipFree(p.d_ptr);
+        if (p.d_idx) (void)hipFree(p.d_idx);
+        fail(BLCO_ENOMEM, "device allocation or upload of the window reduce plan failed");
+        return nullptr;
+    }
+    d->wplans.push_back(p);
+    return &d->wplans.back();
+}
+
+bool lvl3_output_fits(const blco_dev* d, int mode, int rank, int ftype)
 {
     uint64_t esz = ftype == BLCO_F64 ? 8 : 4;
     return d->dims[mode] * (uint64_t)rank * esz <= LDS_OUT_MAX;
@@ -202,7 +339,7 @@ static bool lvl3_fits(const blco_dev* d, int mode, int rank, int ftype)
 int auto_kernel(const blco_dev* d, int mode, int rank, int ftype)
 {
     // src/alto_dev.cu:1975,1996-2001: short modes -> lvl3, else lvl1.
-    if (d->dims[mode] <= (uint64_t)LVL3_MAX_MODE_LENGTH && lvl3_fits(d, mode, rank, ftype))
+    if (d->dims[mode] <= (uint64_t)LVL3_MAX_MODE_LENGTH && lvl3_output_fits(d, mode, rank, ftype))
         return BLCO_K_LVL3;
     return BLCO_K_BATCHED2;
 }
@@ -260,7 +397,16 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     if (nprtn <= 0) nprtn = DEFAULT_NPRTN;
     if (nprtn > 1024) return fail(BLCO_EINVAL, "nprtn %d > 1024", nprtn);
     if (!st) st = d->stream;
-    if (kernel == BLCO_K_AUTO || kernel == BLCO_K_SELECT) kernel = auto_kernel(d, mode, rank, ftype);
+    if (kernel == BLCO_K_SELECT) {
+        int k = 0, np = 0;
+        if (select_kernel(d, mode, rank, ftype, &k, &np)) {
+            kernel = k;
+            if (k == BLCO_K_LVL3) nprtn = np;
+        } else {
+            kernel = BLCO_K_AUTO;
+        }
+    }
+    if (kernel == BLCO_K_AUTO) kernel = auto_kernel(d, mode, rank, ftype);
     int which;
     switch (kernel) {
         case BLCO_K_LVL1:
@@ -269,7 +415,7 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         case BLCO_K_BATCHED2: which = W_WINDOW; break;
         case BLCO_K_LVL3:
             which = W_LVL3;
-            if (!lvl3_fits(d, mode, rank, ftype)) {
+            if (!lvl3_output_fits(d, mode, rank, ftype)) {
                 // the reference warns and proceeds (src/alto_dev.cu:1866); a
                 // privatised copy must fit in LDS here, so lvl1 runs instead
                 which = W_WINDOW;
@@ -312,29 +458,29 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     int rc = BLCO_OK;
     if (which == W_WINDOW) {
         const int acc = ftype == BLCO_F64 ? 8 : 4;
-        const WindowTab* wt = window_table(d, choose_window_k(d, mode, rank, acc));
-        if (!wt) return BLCO_ENOMEM;
+        const WindowPlan* wp = window_plan(d, mode, rank, acc);
+        if (!wp) return BLCO_ENOMEM;
+        const WindowTab* wt = wp->wt;
+        const size_t need = wp->prows * (size_t)rank * acc;
+        if (d->part_bytes < need) {
+            if (d->part) (void)hipFree(d->part);
+            d->part = nullptr;
+            d->part_bytes = 0;
+            HIP_TRY(hipMalloc(&d->part, need));
+            d->part_bytes = need;
+        }
         a.sc = wt->d_sc;
         a.sc_range = wt->d_range + (uint64_t)mode * wt->n * 2;
-        const uint32_t total_rows = (uint32_t)(WINDOW_LDS_BYTES / ((uint64_t)rank * acc));
-        uint32_t hot_rows = (uint32_t)std::min<size_t>(d->hot_list_h[mode].size(), total_rows / 2);
-        if (hot_rows) {
-            const size_t need = (size_t)HOT_PARTS * hot_rows * rank * acc;
-            if (d->hotpart_bytes < need) {
-                if (d->hotpart) (void)hipFree(d->hotpart);
-                d->hotpart = nullptr;
-                d->hotpart_bytes = 0;
-                HIP_TRY(hipMalloc(&d->hotpart, need));
-                HIP_TRY(hipMemset(d->hotpart, 0, need));  // later calls leave it cleared
-                d->hotpart_bytes = need;
-            }
-        }
-        a.hot_rank = hot_rows ? d->d_hot_rank[mode] : nullptr;
-        a.hot_rows = hot_rows;
-        a.hotpart = d->hotpart;
-        a.hot_parts = HOT_PARTS;
-        a.lds_rows = total_rows - hot_rows;
-        const size_t lds = (size_t)total_rows * rank * acc;
+        a.hot_rank = wp->hot_rows ? d->d_hot_rank[mode] : nullptr;
+        a.hot_rows = wp->hot_rows;
+        a.lds_rows = wp->lds_rows;
+        a.sc_poff = wp->d_poff;
+        a.hot_poff = wp->hot_poff;
+        a.part = d->part;
+        const size_t lds = (size_t)(wp->hot_rows + wp->lds_rows) * rank * acc;
+        bool wide = false;  // any mode index >= 2^32 needs 64-bit indices
+        for (int m = 0; m < d->nmodes; ++m) wide |= d->dims[m] > 0xFFFFFFFFull;
+        const int wk = wide ? W_WINDOW64 : W_WINDOW;
         const bool per_block = kernel == BLCO_K_LVL2;
         const int vsz = value_size(d->vdtype);
         if (!d->stream_data) {
@@ -345,12 +491,12 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
                 for (uint64_t b = 0; b < d->nblocks && !rc; ++b) {
                     a.chunk0 = wt->first[b];
                     a.nchunks = wt->first[b + 1] - wt->first[b];
-                    if (a.nchunks) rc = launch_mttkrp(W_WINDOW, a, ftype, a.nchunks, lds, st);
+                    if (a.nchunks) rc = launch_mttkrp(wk, a, ftype, a.nchunks, lds, st);
                 }
             } else {
                 a.chunk0 = 0;
                 a.nchunks = wt->n;
-                if (wt->n) rc = launch_mttkrp(W_WINDOW, a, ftype, wt->n, lds, st);
+                if (wt->n) rc = launch_mttkrp(wk, a, ftype, wt->n, lds, st);
             }
         } else {
             // stream entry ranges of at most slot_nnz nonzeros within a block
@@ -381,16 +527,17 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
                     a.coord_base = begin;
                     a.chunk0 = e;
                     a.nchunks = e1 - e;
-                    rc = launch_mttkrp(W_WINDOW, a, ftype, a.nchunks, lds, st);
+                    rc = launch_mttkrp(wk, a, ftype, a.nchunks, lds, st);
                     HIP_TRY(hipEventRecord(d->ev_done[s], st));
                     ++u;
                     e = e1;
                 }
             }
         }
-        if (!rc && hot_rows)
-            rc = launch_hot_reduce(out, d->hotpart, d->d_hot_list[mode], hot_rows, rank, HOT_PARTS,
-                                   ftype == BLCO_F64 ? BLCO_F64 : BLCO_F32, st);
+        // every partial row was written by its workgroup: fold them in
+        if (!rc)
+            rc = launch_window_reduce(out, d->part, wp->d_rows, wp->d_ptr, wp->d_idx, wp->nred, rank,
+                                      otype, st);
     } else if (!d->stream_data) {
         a.coords = d->d_coords;
         a.vals = d->d_vals;
@@ -534,6 +681,7 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
     const uint64_t nfine = fbeg.size();
     // pass 1: bounds check and per-mode row counts
     std::vector<std::vector<uint32_t>> hot(nmodes);
+    std::vector<std::vector<uint64_t>> hot_cnt(nmodes);
     for (int m = 0; m < nmodes; ++m) {
         std::vector<uint32_t> cnt;
         const bool count = dims[m] <= (1ull << 31);
@@ -560,6 +708,7 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
             });
             if (cand.size() > HOT_EXCLUDE) cand.resize(HOT_EXCLUDE);
             hot[m] = cand;
+            for (uint32_t r : cand) hot_cnt[m].push_back(cnt[r]);
         }
     }
     // pass 2: per fine chunk, per mode, the row range of the non-hot rows
@@ -606,12 +755,17 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
     d->nblocks = nblocks;
     d->bstart.assign(block_start, block_start + nblocks);
     d->bnnz.assign(block_nnz, block_nnz + nblocks);
+    compute_static_features(nmodes, d->dims, d->masks, d->pos, nnz, coords, nblocks, block_start, block_nnz,
+                            block_coords, BLCO_SEL_NFEAT, &d->sfeat[0][0]);
     d->fine_begin.swap(fbeg);
     d->fine_len.swap(flen);
     d->fine_block.swap(fblk);
     d->fine_lo.swap(flo);
     d->fine_hi.swap(fhi);
-    for (int m = 0; m < nmodes; ++m) d->hot_list_h[m] = hot[m];
+    for (int m = 0; m < nmodes; ++m) {
+        d->hot_list_h[m] = hot[m];
+        d->hot_cnt_h[m] = hot_cnt[m];
+    }
     auto cleanup = [&](int rc) { blco_dev_free(d); return rc; };
     if (hipGetDevice(&d->device) != hipSuccess) return cleanup(fail(BLCO_EHIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
@@ -715,7 +869,13 @@ void blco_dev_free(blco_dev* d)
         if (d->d_hot_list[m]) (void)hipFree(d->d_hot_list[m]);
         if (d->d_hot_rank[m]) (void)hipFree(d->d_hot_rank[m]);
     }
-    if (d->hotpart) (void)hipFree(d->hotpart);
+    for (auto& p : d->wplans) {
+        if (p.d_poff) (void)hipFree(p.d_poff);
+        if (p.d_rows) (void)hipFree(p.d_rows);
+        if (p.d_ptr) (void)hipFree(p.d_ptr);
+        if (p.d_idx) (void)hipFree(p.d_idx);
+    }
+    if (d->part) (void)hipFree(d->part);
     if (d->d_bcoords) (void)hipFree(d->d_bcoords);
     if (d->d_coords) (void)hipFree(d->d_coords);
     if (d->d_vals) (void)hipFree(d->d_vals);
@@ -740,6 +900,34 @@ int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype, void* const* 
 {
     return mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, thread_cf,
                   (hipStream_t)stream, nullptr);
+}
+
+int blco_mttkrp_time(blco_dev* d, int mode, int rank, int factor_dtype, void* const* factors, void* out,
+                     int kernel, int nprtn, int reps, float* ms, int* used_kernel)
+{
+    if (!ms || reps < 1) return fail(BLCO_EINVAL, "bad timing arguments");
+    // one untimed call builds the tables and the scratch
+    int rc = mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, 0, nullptr, used_kernel);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(BLCO_EHIP, "event creation failed");
+    }
+    hipError_t e = hipEventRecord(e0, d->stream);
+    for (int r = 0; r < reps && !rc && e == hipSuccess; ++r)
+        rc = mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, 0, nullptr, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, d->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess && !rc) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    HIP_TRY(e);
+    *ms = t / (float)reps;
+    return BLCO_OK;
 }
 
 int blco_auto_kernel(const blco_dev* d, int mode, int rank)

@@ -31,6 +31,28 @@ struct WindowTab {
     std::vector<SuperChunk> h_sc;
 };
 
+// The row-window MTTKRP of one (mode, rank, accumulator size): window table,
+// LDS split, where each workgroup writes its partial rows, and the
+// partial-row lists that the reduce folds per output row.
+struct WindowPlan {
+    int mode = 0, rank = 0, acc = 0;
+    const WindowTab* wt = nullptr;
+    uint32_t lds_rows = 0, hot_rows = 0;
+    uint64_t* d_poff = nullptr;  // [entry + 1]: window of entry e = partial rows [poff[e], poff[e+1])
+    uint64_t hot_poff = 0;       // first partial row of the hot areas
+    uint64_t prows = 0;          // partial rows in all
+    uint64_t nred = 0;           // output rows with at least one partial row
+    uint64_t* d_rows = nullptr;  // [nred] those rows
+    uint64_t* d_ptr = nullptr;   // [nred + 1] CSR into d_idx
+    uint32_t* d_idx = nullptr;   // [prows] partial rows, grouped by output row
+};
+
+// A selector decision for (mode, rank, factor type); kernel 0 = none.
+struct SelCache {
+    int mode, rank, ftype, kernel, nprtn;
+    uint64_t gen;  // selector generation the decision was made with
+};
+
 struct ChunkTab {
     int tpw = 0;
     Chunk* d = nullptr;
@@ -62,14 +84,18 @@ struct blco_dev {
     std::vector<uint64_t> fine_begin;
     std::vector<uint32_t> fine_len, fine_block;
     std::vector<uint64_t> fine_lo, fine_hi;
-    int window_k[BLCO_MAX_MODES][3] = {};  // chosen k + 1 per (mode, accumulator size class)
+    std::list<blco::WindowPlan> wplans;
     // most frequent rows per mode (count >= nnz / 2048, at most HOT_EXCLUDE),
     // in decreasing frequency; fine ranges exclude them
     std::vector<uint32_t> hot_list_h[BLCO_MAX_MODES];
+    std::vector<uint64_t> hot_cnt_h[BLCO_MAX_MODES];  // their nonzero counts
     uint32_t* d_hot_list[BLCO_MAX_MODES] = {};
     int32_t* d_hot_rank[BLCO_MAX_MODES] = {};
-    void* hotpart = nullptr;
-    size_t hotpart_bytes = 0;
+    void* part = nullptr;  // partial rows of the row-window MTTKRP
+    size_t part_bytes = 0;
+    // selector features (rank- and type-independent entries) and decisions
+    float sfeat[BLCO_MAX_MODES][BLCO_SEL_NFEAT] = {};
+    std::vector<blco::SelCache> sel_cache;
     // streaming
     uint64_t* h_coords = nullptr;
     void* h_vals = nullptr;
@@ -90,8 +116,11 @@ namespace blco {
 
 const ChunkTab* chunk_table(blco_dev* d, int tpw);
 const WindowTab* window_table(blco_dev* d, int k);
-int choose_window_k(blco_dev* d, int mode, int rank, int acc_bytes);
+const WindowPlan* window_plan(blco_dev* d, int mode, int rank, int acc_bytes);
 int auto_kernel(const blco_dev* d, int mode, int rank, int ftype);
+bool lvl3_output_fits(const blco_dev* d, int mode, int rank, int ftype);
+int mode_features(const blco_dev* d, int mode, int rank, int ftype, float* out);
+bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int* nprtn);
 int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
            int nprtn, int tpw, hipStream_t st, int* used_kernel);
 

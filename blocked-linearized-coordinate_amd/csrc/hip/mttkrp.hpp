@@ -51,19 +51,21 @@ struct MttkrpArgs {
     uint32_t lds_rows;         // rows the LDS window holds
     const int32_t* hot_rank;   // output mode: frequency rank of a row, -1 if not a candidate
     uint32_t hot_rows;         // ranks < hot_rows are accumulated in the LDS hot area
-    void* hotpart;             // hot_parts x hot_rows x rank partial copies
-    int hot_parts;
+    const uint64_t* sc_poff;   // [entry + 1]: the entry's window is partial rows [poff[e], poff[e+1])
+    uint64_t hot_poff;         // first partial row of the hot areas (entry e: + e * hot_rows)
+    void* part;                // partial rows (rank wide), reduced by k_window_reduce
 };
 
-enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3 };
+enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4 };
 
 constexpr uint32_t FINE_CHUNK = 1024;        // nonzeros per range-table entry
-constexpr uint32_t WINDOW_LDS_BYTES = 48 * 1024;
-constexpr uint32_t HOT_CANDIDATES = 512;     // most frequent rows ranked per mode
-constexpr int HOT_PARTS = 32;                // partial copies of the hot rows
+constexpr uint32_t WINDOW_LDS_BYTES = 40 * 1024;  // hot area + row window per workgroup
+constexpr int WIN_WAVES = 8;                      // waves per row-window workgroup
+constexpr int WIN_BATCH = 4;                      // positions whose gathers are issued together
 
-int launch_hot_reduce(void* out, void* hotpart, const uint32_t* hot_list, uint32_t hot_rows,
-                      int rank, int parts, int otype, hipStream_t st);
+// out[rows[u]][:] += sum of the partial rows idx[ptr[u] .. ptr[u+1]).
+int launch_window_reduce(void* out, const void* part, const uint64_t* rows, const uint64_t* ptr,
+                         const uint32_t* idx, uint64_t nrows, int rank, int otype, hipStream_t st);
 
 const void* mttkrp_kernel(int which, int nmodes, int ftype, int rank, int* C, int* L);
 int launch_mttkrp(int which, const MttkrpArgs& a, int ftype, uint32_t grid, size_t dyn_lds,

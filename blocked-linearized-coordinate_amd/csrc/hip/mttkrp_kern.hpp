@@ -1,0 +1,522 @@
+// mttkrp_kern.hpp -- the MTTKRP kernel templates (lvl1, lvl3, row window,
+// atomic) and their per-shape selection; each precision instantiates them in
+// its own translation unit (mttkrp_f64.hip, mttkrp_f32.hip, mttkrp_bf16.hip).
+#pragma once
+
+#include "devcommon.hpp"
+#include "mttkrp.hpp"
+
+namespace blco {
+
+// ------------------------------------------------------------ row loads
+
+template <typename FT, int C>
+__device__ __forceinline__ void load_row(const FT* __restrict__ p,
+                                         typename Acc<FT>::type (&x)[C])
+{
+    using AT = typename Acc<FT>::type;
+    if constexpr (C * sizeof(FT) == 16) {
+        uint4 u = *reinterpret_cast<const uint4*>(p);
+        const FT* e = reinterpret_cast<const FT*>(&u);
+        #pragma unroll
+        for (int c = 0; c < C; ++c) x[c] = to_acc(e[c]);
+    } else if constexpr (C * sizeof(FT) == 8) {
+        uint2 u = *reinterpret_cast<const uint2*>(p);
+        const FT* e = reinterpret_cast<const FT*>(&u);
+        #pragma unroll
+        for (int c = 0; c < C; ++c) x[c] = to_acc(e[c]);
+    } else {
+        #pragma unroll
+        for (int c = 0; c < C; ++c) x[c] = (AT)to_acc(p[c]);
+    }
+}
+
+template <typename AT>
+__device__ __forceinline__ void atomic_add_global(AT* p, AT v)
+{
+    unsafeAtomicAdd(p, v);
+}
+
+template <typename AT>
+__device__ __forceinline__ void atomic_add_lds(AT* p, AT v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ascending bitonic sort of one 32-bit key per lane across the wave.
+__device__ __forceinline__ uint32_t wave_sort64(uint32_t key, int lane)
+{
+    #pragma unroll
+    for (int k = 2; k <= WAVE; k <<= 1) {
+        #pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint32_t other = (uint32_t)__shfl_xor((int)key, j);
+            bool up = (lane & k) == 0;
+            bool lower = (lane & j) == 0;
+            uint32_t mn = key < other ? key : other;
+            uint32_t mx = key < other ? other : key;
+            key = (lower == up) ? mn : mx;
+        }
+    }
+    return key;
+}
+
+// Loads, decodes and sorts one tile; leaves the sorted entries in the
+// wave's LDS strip (s_idx[m][skewed position], s_val[...]).
+template <int N, int L, typename VT, typename AT>
+__device__ __forceinline__ void stage_tile(const MttkrpArgs& a, const uint64_t (&bc)[N],
+                                           uint64_t e0, uint32_t nvalid, int lane,
+                                           uint64_t (*s_idx)[WAVE + WAVE / L],
+                                           AT* s_val)
+{
+    const bool valid = (uint32_t)lane < nvalid;
+    const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    uint64_t coord = valid ? a.coords[e0 + lane - a.coord_base] : 0;
+    AT v = valid ? (AT)vals[e0 + lane - a.coord_base] : (AT)0;
+    uint64_t idx[N];
+    uint64_t orow = 0;
+    #pragma unroll
+    for (int m = 0; m < N; ++m) {
+        idx[m] = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
+        if (m == a.mode) orow = idx[m];
+    }
+    uint64_t base = __shfl(orow, 0);
+    uint64_t diff = orow - base + (1ull << 25);
+    bool fits = diff < (1ull << 26);
+    uint32_t key;
+    if (__ballot(valid && !fits) == 0) {
+        key = valid ? (((uint32_t)diff << 6) | (uint32_t)lane) : 0xFFFFFFFFu;
+        key = wave_sort64(key, lane);
+    } else {
+        key = (uint32_t)lane;  // rows too far apart: keep tile order
+    }
+    const int src = (int)(key & 63u);
+    const int q = lane + lane / L;
+    #pragma unroll
+    for (int m = 0; m < N; ++m) s_idx[m][q] = __shfl(idx[m], src);
+    s_val[q] = __shfl(v, src);
+    wave_lds_sync();
+}
+
+// Walks the sorted tile: slot `slot` covers positions [slot*L, slot*L + L).
+// FLUSH(row, acc) is called once per segment.
+template <int N, typename FT, int C, int L, bool EXACT, typename Flush>
+__device__ __forceinline__ void segment_tile(const MttkrpArgs& a, uint32_t nvalid, int slot,
+                                             int col, uint64_t (*s_idx)[WAVE + WAVE / L],
+                                             const typename Acc<FT>::type* s_val, Flush flush)
+{
+    using AT = typename Acc<FT>::type;
+    const int R = a.rank, mode = a.mode;
+    int kmax = (int)nvalid - slot * L;
+    kmax = kmax < 0 ? 0 : (kmax > L ? L : kmax);
+    const int qb = slot * (L + 1);
+    AT acc[C];
+    #pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = 0;
+    uint64_t cur = 0;
+    bool have = false;
+    for (int k = 0; k < kmax; ++k) {
+        const int qq = qb + k;
+        const uint64_t r = s_idx[mode][qq];
+        if (have && r != cur) {
+            flush(cur, acc);
+            #pragma unroll
+            for (int c = 0; c < C; ++c) acc[c] = 0;
+        }
+        cur = r;
+        have = true;
+        AT p[C];
+        const AT vv = s_val[qq];
+        #pragma unroll
+        for (int c = 0; c < C; ++c) p[c] = vv;
+        #pragma unroll
+        for (int m = 0; m < N; ++m) {
+            if (m == mode) continue;
+            const FT* f = reinterpret_cast<const FT*>(a.f[m]) + s_idx[m][qq] * (uint64_t)R + col;
+            AT x[C];
+            if (EXACT || col + C <= R) {
+                load_row<FT, C>(f, x);
+            } else {
+                #pragma unroll
+                for (int c = 0; c < C; ++c) x[c] = (col + c < R) ? (AT)to_acc(f[c]) : (AT)0;
+            }
+            #pragma unroll
+            for (int c = 0; c < C; ++c) p[c] *= x[c];
+        }
+        #pragma unroll
+        for (int c = 0; c < C; ++c) acc[c] += p[c];
+    }
+    if (have) flush(cur, acc);
+}
+
+// --------------------------------------------------------------- lvl1
+
+template <int N, typename FT, typename VT, int C, int L, bool EXACT>
+__global__ __launch_bounds__(WG) void k_mttkrp_lvl1(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    __shared__ uint64_t s_idx[WAVES_PER_WG][N][WAVE + WAVE / L];
+    __shared__ AT s_val[WAVES_PER_WG][WAVE + WAVE / L];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = a.chunk0 + xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
+    if (c >= a.chunk0 + a.nchunks) return;  // whole wave leaves; no block barriers below
+    const Chunk ch = a.chunks[c];
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const int R = a.rank;
+    for (uint32_t t0 = 0; t0 < ch.len; t0 += WAVE) {
+        const uint32_t nvalid = min((uint32_t)WAVE, ch.len - t0);
+        stage_tile<N, L, VT, AT>(a, bc, ch.begin + t0, nvalid, lane, s_idx[w], s_val[w]);
+        for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+            const int col = cb + sub * C;
+            segment_tile<N, FT, C, L, EXACT>(a, nvalid, slot, col, s_idx[w], s_val[w],
+                [&](uint64_t row, const AT (&acc)[C]) {
+                    AT* o = out + row * (uint64_t)R + col;
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc)
+                        if (EXACT || col + cc < R) atomic_add_global(o + cc, acc[cc]);
+                });
+        }
+        wave_lds_sync();
+    }
+}
+
+// --------------------------------------------------------------- lvl3
+
+template <int N, typename FT, typename VT, int C, int L, bool EXACT>
+__global__ __launch_bounds__(WG) void k_mttkrp_lvl3(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    __shared__ uint64_t s_idx[WAVES_PER_WG][N][WAVE + WAVE / L];
+    __shared__ AT s_val[WAVES_PER_WG][WAVE + WAVE / L];
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    AT* s_out = reinterpret_cast<AT*>(s_dyn);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int R = a.rank;
+    const uint64_t cells = a.out_rows * (uint64_t)R;
+    for (uint64_t e = threadIdx.x; e < cells; e += WG) s_out[e] = 0;
+    __syncthreads();
+    const uint32_t g = blockIdx.x;
+    const uint32_t cbeg = a.chunk0 + g * a.chunks_per_wg;
+    const uint32_t cend = min(cbeg + a.chunks_per_wg, a.chunk0 + a.nchunks);
+    const int slot = lane / L, sub = lane % L;
+    for (uint32_t c = cbeg + w; c < cend; c += WAVES_PER_WG) {
+        const Chunk ch = a.chunks[c];
+        uint64_t bc[N];
+        #pragma unroll
+        for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+        for (uint32_t t0 = 0; t0 < ch.len; t0 += WAVE) {
+            const uint32_t nvalid = min((uint32_t)WAVE, ch.len - t0);
+            stage_tile<N, L, VT, AT>(a, bc, ch.begin + t0, nvalid, lane, s_idx[w], s_val[w]);
+            for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+                const int col = cb + sub * C;
+                segment_tile<N, FT, C, L, EXACT>(a, nvalid, slot, col, s_idx[w], s_val[w],
+                    [&](uint64_t row, const AT (&acc)[C]) {
+                        AT* o = s_out + row * (uint64_t)R + col;
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc)
+                            if (EXACT || col + cc < R) atomic_add_lds(o + cc, acc[cc]);
+                    });
+            }
+            wave_lds_sync();
+        }
+    }
+    __syncthreads();
+    AT* dst = reinterpret_cast<AT*>(a.nprtn > 1 ? a.partials : a.out);
+    if (a.nprtn > 1) dst += (uint64_t)(g % (uint32_t)a.nprtn) * cells;
+    for (uint64_t e = threadIdx.x; e < cells; e += WG) {
+        AT v = s_out[e];
+        if (v != (AT)0) atomic_add_global(dst + e, v);
+    }
+}
+
+// --------------------------------------------------------- row window
+
+// One workgroup of WIN_WAVES waves per super-chunk.  LDS holds only the two
+// accumulators: the hot area (the output mode's most frequent rows, by
+// frequency rank) and the row window [lo, lo + lds_rows) of the super-chunk's
+// other rows.  Tiles are decoded and sorted in registers; the segment walk
+// reads the sorted entries with cross-lane permutes and issues the factor-row
+// gathers of WIN_BATCH positions before consuming them.  A segment goes to
+// the hot area, else to the window, else straight to HBM (atomic).  At the end
+// the window and the hot area are written, whole, with plain coalesced stores
+// to the workgroup's own partial rows; k_window_reduce adds the partial rows
+// of every output row into the output.  On a multi-XCD device an atomic is
+// performed beyond the XCD's L2, so the flush avoids atomics altogether.
+template <int N, typename FT, typename VT, int C, int L, bool EXACT, typename IT>
+__global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    constexpr int PB = L < WIN_BATCH ? L : WIN_BATCH;
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const int R = a.rank, mode = a.mode;
+    AT* s_hot = reinterpret_cast<AT*>(s_dyn);
+    AT* s_win = s_hot + (uint64_t)a.hot_rows * R;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t e = a.chunk0 + g;  // grid == entry count
+    const SuperChunk sc = a.sc[e];
+    const uint64_t lo = a.sc_range[2 * (uint64_t)e];
+    const uint64_t wrows = a.sc_poff[e + 1] - a.sc_poff[e];  // <= lds_rows; 0 = no window
+    const uint64_t cells = ((uint64_t)a.hot_rows + wrows) * R;
+    for (uint64_t i = threadIdx.x; i < cells; i += WIN_WAVES * WAVE) s_hot[i] = (AT)0;
+    __syncthreads();
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)sc.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    const uint32_t ntiles = (sc.len + WAVE - 1) / WAVE;
+
+    auto flush = [&](uint64_t row, int col, const AT (&acc)[C]) {
+        const int32_t hr = a.hot_rank ? a.hot_rank[row] : -1;
+        AT* o;
+        bool lds = true;
+        if (hr >= 0 && (uint32_t)hr < a.hot_rows) o = s_hot + (uint64_t)hr * R + col;
+        else if (row >= lo && row - lo < wrows) o = s_win + (row - lo) * (uint64_t)R + col;
+        else { o = out + row * (uint64_t)R + col; lds = false; }
+        #pragma unroll
+        for (int cc = 0; cc < C; ++cc) {
+            if (!(EXACT || col + cc < R)) continue;
+            if (lds) atomic_add_lds(o + cc, acc[cc]);
+            else atomic_add_global(o + cc, acc[cc]);
+        }
+    };
+
+    for (uint32_t t = w; t < ntiles; t += WIN_WAVES) {
+        const uint32_t t0 = t * WAVE;
+        const uint32_t nvalid = min((uint32_t)WAVE, sc.len - t0);
+        // decode and sort the tile by output row, all in registers
+        const bool valid = (uint32_t)lane < nvalid;
+        const uint64_t el = sc.begin + t0 + lane - a.coord_base;
+        const uint64_t coord = valid ? a.coords[el] : 0;
+        const AT v = valid ? (AT)vals[el] : (AT)0;
+        IT idx[N];
+        IT orow = 0;
+        #pragma unroll
+        for (int m = 0; m < N; ++m) {
+            idx[m] = (IT)(((coord >> a.pos[m]) & a.mask[m]) | bc[m]);
+            if (m == mode) orow = idx[m];
+        }
+        const IT base = __shfl(orow, 0);
+        const uint64_t diff = (uint64_t)orow - (uint64_t)base + (1ull << 25);
+        uint32_t key;
+        if (__ballot(valid && diff >= (1ull << 26)) == 0) {
+            key = valid ? (((uint32_t)diff << 6) | (uint32_t)lane) : 0xFFFFFFFFu;
+            key = wave_sort64(key, lane);
+        } else {
+            key = (uint32_t)lane;
+        }
+        const int src = (int)(key & 63u);
+        IT sidx[N];
+        #pragma unroll
+        for (int m = 0; m < N; ++m) sidx[m] = __shfl(idx[m], src);
+        const AT sv = __shfl(v, src);
+        IT srow = 0;
+        #pragma unroll
+        for (int m = 0; m < N; ++m)
+            if (m == mode) srow = sidx[m];
+        // segment walk over positions [slot*L, slot*L + L)
+        int kmax = (int)nvalid - slot * L;
+        kmax = kmax < 0 ? 0 : (kmax > L ? L : kmax);
+        for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+            const int col = cb + sub * C;
+            AT acc[C];
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
+            uint64_t cur = 0;
+            bool have = false;
+            #pragma unroll
+            for (int k0 = 0; k0 < L; k0 += PB) {
+                IT rr[PB];
+                AT pv[PB][C];
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    const int q = slot * L + k0 + b;
+                    rr[b] = __shfl(srow, q);
+                    const AT vq = __shfl(sv, q);
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) pv[b][cc] = vq;
+                    #pragma unroll
+                    for (int m = 0; m < N; ++m) {
+                        if (m == mode) continue;
+                        const IT im = __shfl(sidx[m], q);
+                        AT x[C];
+                        if (k0 + b < kmax) {
+                            const FT* f = reinterpret_cast<const FT*>(a.f[m]) + (uint64_t)im * R + col;
+                            if (EXACT || col + C <= R) {
+                                load_row<FT, C>(f, x);
+                            } else {
+                                #pragma unroll
+                                for (int cc = 0; cc < C; ++cc) x[cc] = (col + cc < R) ? (AT)to_acc(f[cc]) : (AT)0;
+                            }
+                        } else {
+                            #pragma unroll
+                            for (int cc = 0; cc < C; ++cc) x[cc] = 0;
+                        }
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) pv[b][cc] *= x[cc];
+                    }
+                }
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    if (k0 + b >= kmax) break;
+                    if (have && (uint64_t)rr[b] != cur) {
+                        flush(cur, col, acc);
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
+                    }
+                    cur = rr[b];
+                    have = true;
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) acc[cc] += pv[b][cc];
+                }
+            }
+            if (have) flush(cur, col, acc);
+        }
+    }
+    __syncthreads();
+    AT* part = reinterpret_cast<AT*>(a.part);
+    const uint64_t hcells = (uint64_t)a.hot_rows * R;
+    if (hcells) {
+        AT* hp = part + (a.hot_poff + (uint64_t)e * a.hot_rows) * R;
+        for (uint64_t i = threadIdx.x; i < hcells; i += WIN_WAVES * WAVE) hp[i] = s_hot[i];
+    }
+    if (wrows) {
+        AT* wp = part + a.sc_poff[e] * R;
+        for (uint64_t i = threadIdx.x; i < wrows * R; i += WIN_WAVES * WAVE) wp[i] = s_win[i];
+    }
+}
+
+// ------------------------------------------------------------- atomic
+
+template <int N, typename FT, typename VT, int C, int L, bool EXACT>
+__global__ __launch_bounds__(WG) void k_mttkrp_atomic(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = a.chunk0 + xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
+    if (c >= a.chunk0 + a.nchunks) return;
+    const Chunk ch = a.chunks[c];
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    constexpr int S = WAVE / L;
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    const int R = a.rank, mode = a.mode;
+    for (uint32_t t0 = 0; t0 < ch.len; t0 += WAVE) {
+        const uint32_t nvalid = min((uint32_t)WAVE, ch.len - t0);
+        const bool valid = (uint32_t)lane < nvalid;
+        const uint64_t e = ch.begin + t0 + lane - a.coord_base;
+        const uint64_t coord = valid ? a.coords[e] : 0;
+        const AT v = valid ? (AT)vals[e] : (AT)0;
+        uint64_t idx[N];
+        #pragma unroll
+        for (int m = 0; m < N; ++m) idx[m] = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
+        for (int k = 0; k < L; ++k) {
+            const int q = k * S + slot;  // nonzero handled by this slot in step k
+            uint64_t rows[N];
+            #pragma unroll
+            for (int m = 0; m < N; ++m) rows[m] = __shfl(idx[m], q);
+            const AT vv = __shfl(v, q);
+            if ((uint32_t)q >= nvalid) continue;
+            for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+                const int col = cb + sub * C;
+                AT p[C];
+                #pragma unroll
+                for (int cc = 0; cc < C; ++cc) p[cc] = vv;
+                uint64_t orow = 0;
+                #pragma unroll
+                for (int m = 0; m < N; ++m) {
+                    if (m == mode) { orow = rows[m]; continue; }
+                    const FT* f = reinterpret_cast<const FT*>(a.f[m]) + rows[m] * (uint64_t)R + col;
+                    AT x[C];
+                    if (EXACT || col + C <= R) {
+                        load_row<FT, C>(f, x);
+                    } else {
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) x[cc] = (col + cc < R) ? (AT)to_acc(f[cc]) : (AT)0;
+                    }
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) p[cc] *= x[cc];
+                }
+                AT* o = out + orow * (uint64_t)R + col;
+                #pragma unroll
+                for (int cc = 0; cc < C; ++cc)
+                    if (EXACT || col + cc < R) atomic_add_global(o + cc, p[cc]);
+            }
+        }
+    }
+}
+
+// Column layout (C columns per lane, L lanes per nonzero) for a rank and
+// element size; EXACT when C * L == rank.
+struct Shape {
+    int C, L;
+    bool exact;
+};
+
+template <int N, typename FT, typename VT, int C, int L, bool EX>
+inline const void* kernel_ptr(int which)
+{
+    if (which == W_LVL1) return (const void*)&k_mttkrp_lvl1<N, FT, VT, C, L, EX>;
+    if (which == W_LVL3) return (const void*)&k_mttkrp_lvl3<N, FT, VT, C, L, EX>;
+    if (which == W_WINDOW) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint32_t>;
+    if (which == W_WINDOW64) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint64_t>;
+    return (const void*)&k_mttkrp_atomic<N, FT, VT, C, L, EX>;
+}
+
+template <int N, typename FT, typename VT>
+inline const void* select_shape(const Shape& s, int which)
+{
+    if (s.exact) {
+        if (s.C == 1 && s.L == 8) return kernel_ptr<N, FT, VT, 1, 8, true>(which);
+        if (s.C == 2 && s.L == 8) return kernel_ptr<N, FT, VT, 2, 8, true>(which);
+        if constexpr (sizeof(FT) == 8) {
+            if (s.C == 2 && s.L == 16) return kernel_ptr<N, FT, VT, 2, 16, true>(which);
+            if (s.C == 2 && s.L == 32) return kernel_ptr<N, FT, VT, 2, 32, true>(which);
+        } else if constexpr (sizeof(FT) == 4) {
+            if (s.C == 4 && s.L == 8) return kernel_ptr<N, FT, VT, 4, 8, true>(which);
+            if (s.C == 4 && s.L == 16) return kernel_ptr<N, FT, VT, 4, 16, true>(which);
+        } else {
+            if (s.C == 4 && s.L == 8) return kernel_ptr<N, FT, VT, 4, 8, true>(which);
+            if (s.C == 8 && s.L == 8) return kernel_ptr<N, FT, VT, 8, 8, true>(which);
+        }
+        return nullptr;
+    }
+    if (s.L == 8) return kernel_ptr<N, FT, VT, 1, 8, false>(which);
+    if (s.L == 16) return kernel_ptr<N, FT, VT, 1, 16, false>(which);
+    if (s.L == 32) return kernel_ptr<N, FT, VT, 1, 32, false>(which);
+    return kernel_ptr<N, FT, VT, 1, 64, false>(which);
+}
+
+template <typename FT, typename VT>
+inline const void* select_n(int N, const Shape& s, int which)
+{
+    switch (N) {
+        case 2: return select_shape<2, FT, VT>(s, which);
+        case 3: return select_shape<3, FT, VT>(s, which);
+        case 4: return select_shape<4, FT, VT>(s, which);
+        case 5: return select_shape<5, FT, VT>(s, which);
+        default: return nullptr;
+    }
+}
+
+const void* mttkrp_kernel_f64(int N, const Shape& s, int which);
+const void* mttkrp_kernel_f32(int N, const Shape& s, int which);
+const void* mttkrp_kernel_bf16(int N, const Shape& s, int which);
+
+}  // namespace blco

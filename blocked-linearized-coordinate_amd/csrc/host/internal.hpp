@@ -35,6 +35,12 @@ struct Layout {
 // Computes the ALTO / BLCO layout of a shape; returns BLCO_OK or an error.
 int make_layout(int nmodes, const uint64_t* dims, Layout* L);
 
+// Kernel-selector features of every mode that depend on the tensor only
+// (entries 0-3, 5, 7-22 of each row of `stride` floats; features.cpp).
+void compute_static_features(int N, const uint64_t* dims, const uint64_t* masks, const int* pos, uint64_t nnz,
+                             const uint64_t* coords, uint64_t nblocks, const uint64_t* block_start,
+                             const uint64_t* block_nnz, const uint64_t* block_coords, int stride, float* out);
+
 }  // namespace blco
 
 struct blco_coo {

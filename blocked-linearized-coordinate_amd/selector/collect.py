@@ -1,0 +1,134 @@
+"""Timing data for the learned MTTKRP kernel selector.
+
+Tensor t of the selector set (t = 0..399 training, 400..499 held out) is
+drawn from seed 1004 * 100000 + t: order uniform in {3, 4, 5}; mode lengths
+log-uniform in [1e2, 1e7]; nonzeros log-uniform in [1e5, nnz_max]
+(default 1e7; duplicates merged, no top-up); per-mode Zipf exponent uniform
+in [0, 1.5]; rank from {8, 16, 32, 64}; FP32 values (Poisson counts of a
+rank-8 model).  For every mode, every candidate that can run is timed with
+HIP events (blco_mttkrp_time) on factors of ones, and the features of the
+mode are recorded.
+
+  python collect.py --first 0 --count 100 --out data/part0.npz
+
+Writes the file after every tensor, so an interrupted run keeps what it
+measured.  Prints one line per tensor.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("blco", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    B = importlib.util.module_from_spec(spec)
+    sys.modules["blco"] = B
+    spec.loader.exec_module(B)
+    return B
+
+
+BASE_SEED = 1004 * 100000
+
+
+def tensor_spec(t, nnz_max):
+    rng = np.random.default_rng(BASE_SEED + t)
+    order = int(rng.integers(3, 6))
+    dims = [int(round(10 ** rng.uniform(2, 7))) for _ in range(order)]
+    nnz = int(round(10 ** rng.uniform(5, np.log10(nnz_max))))
+    zipf = [float(rng.uniform(0, 1.5)) for _ in range(order)]
+    rank = int(rng.choice([8, 16, 32, 64]))
+    return dict(t=t, seed=BASE_SEED + t, dims=dims, nnz=nnz, zipf=zipf, rank=rank)
+
+
+def ones(B, rows, rank):
+    """f32 device factor of ones (uploaded without a float64 copy)."""
+    a = B.DeviceArray((rows, rank), B.F32)
+    h = np.ones((rows, rank), np.float32)
+    B._check(B.lib().blco_memcpy_htod(a.ptr, h.ctypes.data, a.nbytes))
+    return a
+
+
+def rule_candidate(B, dev, mode, rank, nprtn_rule):
+    k = dev.auto_kernel(mode, rank)
+    for c in range(B.SEL_NCAND):
+        kc, nc = B.sel_candidate(c)
+        if kc == k and (k != B.K_LVL3 or nc == nprtn_rule):
+            return c
+    raise RuntimeError("rule kernel %d is not a candidate" % k)
+
+
+def time_candidate(B, dev, mode, rank, F, out, c):
+    k, np_ = B.sel_candidate(c)
+    ms, used = B.mttkrp_time(dev, mode, rank, F, out, B.F32, k, np_, reps=1)
+    if used != k:
+        raise RuntimeError("candidate %d ran kernel %d" % (c, used))
+    if ms < 20.0:  # short: average more launches
+        ms, _ = B.mttkrp_time(dev, mode, rank, F, out, B.F32, k, np_, reps=5)
+    return ms
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=500)
+    ap.add_argument("--nnz-max", type=float, default=1e7)
+    ap.add_argument("--nprtn-rule", type=int, default=16,
+                    help="lvl3 partitions of the -k 10 rule (the reference's default is the "
+                         "OpenMP thread count)")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+    B = load_pkg()
+    if B.device_count() < 1:
+        sys.exit("collect.py needs a GPU")
+    rows = dict(feat=[], times=[], tensor=[], mode=[], rank=[], rule=[])
+    meta = []
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    for t in range(args.first, args.first + args.count):
+        sp = tensor_spec(t, args.nnz_max)
+        t0 = time.perf_counter()
+        coo = B.Coo.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                             seed=sp["seed"], topup=False)
+        h = B.encode(coo)
+        dev = B.DeviceTensor(h, value_dtype=1)
+        R = sp["rank"]
+        F = [ones(B, d, R) for d in sp["dims"]]
+        tgen = time.perf_counter() - t0
+        for m in range(len(sp["dims"])):
+            out = B.DeviceArray((sp["dims"][m], R), B.F32)
+            tm = np.full(B.SEL_NCAND, np.nan)
+            for c in range(B.SEL_NCAND):
+                if dev.candidate_valid(m, R, B.F32, c):
+                    tm[c] = time_candidate(B, dev, m, R, F, out, c)
+            rows["feat"].append(dev.features(m, R, B.F32))
+            rows["times"].append(tm)
+            rows["tensor"].append(t)
+            rows["mode"].append(m)
+            rows["rank"].append(R)
+            rows["rule"].append(rule_candidate(B, dev, m, R, args.nprtn_rule))
+            out.free()
+        sp.update(nnz_unique=int(coo.nnz), blocks=int(h.nblocks))
+        meta.append(sp)
+        for f in F:
+            f.free()
+        dev.free()
+        np.savez(args.out, feat=np.array(rows["feat"], np.float32), times=np.array(rows["times"]),
+                 tensor=np.array(rows["tensor"]), mode=np.array(rows["mode"]),
+                 rank=np.array(rows["rank"]), rule=np.array(rows["rule"]),
+                 meta=np.array(json.dumps(meta)))
+        best = [np.nanmin(x) for x in rows["times"][-len(sp["dims"]):]]
+        print("tensor %d: order %d nnz %d blocks %d rank %d setup %.1f s total %.1f s best ms %s"
+              % (t, len(sp["dims"]), coo.nnz, h.nblocks, R, tgen, time.perf_counter() - t0,
+                 np.round(best, 3).tolist()), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,13 +26,18 @@ enum blco_dtype { BLCO_F64 = 0, BLCO_F32 = 1, BLCO_BF16 = 2 };
  * src/alto_dev.cu:1954-2032). */
 enum blco_kernel {
     BLCO_K_LVL1 = 1,       /* wave64 segmented reduction, one launch per block */
-    BLCO_K_LVL2 = 2,       /* accepted, runs lvl1 */
-    BLCO_K_LVL3 = 3,       /* LDS-privatised output + nprtn partial matrices */
-    BLCO_K_AUTO = 10,      /* the reference heuristic */
-    BLCO_K_BATCHED = 11,   /* lvl1, all blocks in one launch */
-    BLCO_K_BATCHED2 = 12,  /* same as 11 (work list) */
+    BLCO_K_LVL2 = 2,       /* row window (LDS window + hot rows, partial rows
+                              folded by a reduce), one launch per block; takes
+                              the place of the reference's lvl2 stash kernel */
+    BLCO_K_LVL3 = 3,       /* LDS-privatised output + nprtn partial matrices;
+                              modes too long for LDS run kernel 11 */
+    BLCO_K_AUTO = 10,      /* the reference rule: I_n <= 100 (and fits) -> 3,
+                              else 12 */
+    BLCO_K_BATCHED = 11,   /* segmented lvl1, all blocks in one launch */
+    BLCO_K_BATCHED2 = 12,  /* row window, all blocks in one launch */
     BLCO_K_ATOMIC = 13,    /* one HBM atomic per (nonzero, column) */
-    BLCO_K_SELECT = 20     /* per-mode choice by the installed selector */
+    BLCO_K_SELECT = 20     /* per-mode choice by the installed selector
+                              (the rule of 10 when none is installed) */
 };
 
 /* Number of visible GPUs (0 when none); never initialises a context on a
@@ -178,6 +183,73 @@ void blco_cpals_free(blco_cpals_state* s);
 int blco_cpals(blco_dev* d, const blco_cpals_opts* opts, double* const* U,
                double* lambda, double* fit_trace, int* iters,
                double* final_fit);
+
+/* Event-timed MTTKRP: one untimed call (builds tables and scratch), then
+ * `reps` calls timed with HIP events on the tensor's own stream; *ms is the
+ * mean time of one call.  The output accumulates reps + 1 results.  The
+ * reference's -p benchmark (src/main.cpp:336-378, src/alto_dev.cu:1856-1952)
+ * times the same way; this is also the timing source of the selector's
+ * training data. */
+int blco_mttkrp_time(blco_dev* d, int mode, int rank, int factor_dtype,
+                     void* const* factors, void* out, int kernel, int nprtn,
+                     int reps, float* ms, int* used_kernel);
+
+/* ---- a small MLP run and trained with the library's own HIP kernels ----
+ * (no reference counterpart: the reference's kernel choice is the fixed
+ * rule of src/alto_dev.cu:1975,1996-2001).  Layers are dense, ReLU between
+ * hidden layers and a linear output; widths[0] inputs, widths[nlayers]
+ * outputs.  Parameters are flat f32, layer by layer: W (out x in, row-major)
+ * then b.  Host arrays are row-major, one row per example. */
+#define BLCO_MLP_MAX_LAYERS 4
+#define BLCO_MLP_MAX_WIDTH 256
+typedef struct blco_mlp blco_mlp;
+/* He-uniform weights from the counter generator keyed by seed, zero biases. */
+int blco_mlp_create(int nlayers, const int* widths, uint64_t seed, blco_mlp** out);
+void blco_mlp_free(blco_mlp* h);
+uint64_t blco_mlp_nparams(const blco_mlp* h);
+/* Copies parameters and/or the last gradients (either may be NULL). */
+int blco_mlp_get_params(const blco_mlp* h, float* params, float* grads);
+/* Sets the parameters and resets the Adam state. */
+int blco_mlp_set_params(blco_mlp* h, const float* params);
+int blco_mlp_forward(blco_mlp* h, const float* X, int batch, float* Y);
+/* Masked mean squared error sum(M (Y - T)^2) / sum(M) of the batch and its
+ * gradient with respect to every parameter (kept on the device). */
+int blco_mlp_loss_grad(blco_mlp* h, const float* X, const float* T,
+                       const float* M, int batch, float* loss);
+/* One Adam step (bias-corrected) with the last gradients. */
+int blco_mlp_adam_step(blco_mlp* h, float lr, float beta1, float beta2, float eps);
+/* Mini-batch Adam (0.9, 0.999, 1e-8) over n examples for `epochs` epochs,
+ * shuffled by the counter generator keyed by seed; loss_trace (epochs
+ * entries, may be NULL) gets each epoch's mean batch loss. */
+int blco_mlp_train(blco_mlp* h, const float* X, const float* T, const float* M,
+                   int n, int epochs, int batch, float lr, uint64_t seed,
+                   float* loss_trace);
+
+/* ---- learned MTTKRP kernel selector (kernel id 20) ----
+ * Candidates c = 0..BLCO_SEL_NCAND-1: 0 row window (12), 1 segmented lvl1
+ * (11), 2 atomic (13), 3..9 lvl3 with 1, 2, 4, 8, 16, 32, 64 partitions.
+ * The network maps BLCO_SEL_NFEAT features (normalised by the installed mean
+ * and scale) to predicted log2 times of every candidate; the smallest
+ * prediction among the candidates that can run is taken.  With no network
+ * installed, kernel 20 is kernel 10.  A selector file next to the library
+ * (lib/../selector/selector.bin, or $BLCO_SELECTOR) is installed on first
+ * use. */
+#define BLCO_SEL_NFEAT 24
+#define BLCO_SEL_NCAND 10
+int blco_sel_candidate(int c, int* kernel, int* nprtn);
+/* 1 when candidate c can run for (mode, rank, factor type), else 0. */
+int blco_sel_valid(const blco_dev* d, int mode, int rank, int factor_dtype, int c);
+int blco_mode_features(const blco_dev* d, int mode, int rank, int factor_dtype,
+                       float* feat);
+/* nlayers = 0 uninstalls. */
+int blco_selector_install(int nlayers, const int* widths, const float* params,
+                          const float* xmean, const float* xstd);
+int blco_selector_load(const char* path);
+int blco_selector_installed(void);
+/* The kernel id 20 would run for (mode, rank, factor type) and its lvl3
+ * partition count (0 = the caller's). */
+int blco_select_kernel(blco_dev* d, int mode, int rank, int factor_dtype,
+                       int* kernel, int* nprtn);
 
 #ifdef __cplusplus
 }

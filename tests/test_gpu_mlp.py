@@ -1,0 +1,141 @@
+"""The library's HIP network kernels (forward, masked MSE, backward, Adam)
+against the float64 CPU oracle (oracle/mlp.py), and the learned kernel
+selector's plumbing through the C ABI.
+
+Tolerances: the kernels compute in float32 with fused multiply-adds over at
+most 4096 products, so forward outputs, the loss and every gradient entry
+are compared at 2e-5 relative to the largest magnitude of that quantity.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp as Morc  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+WIDTHS = [24, 64, 64, 10]
+
+
+def batch(widths, n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, widths[0])).astype(np.float32)
+    T = rng.normal(size=(n, widths[-1])).astype(np.float32)
+    Mk = (rng.random((n, widths[-1])) < 0.8).astype(np.float32)
+    return X, T, Mk
+
+
+def close(a, b, tol=2e-5):
+    scale = max(1.0, float(np.abs(b).max()))
+    return float(np.abs(np.asarray(a, np.float64) - b).max()) <= tol * scale
+
+
+@pytest.mark.parametrize("widths", [WIDTHS, [7, 5], [24, 128, 10], [3, 256, 17, 9, 2]])
+def test_forward_and_gradient_match_oracle(B, gpu, widths):
+    net = B.Mlp(widths, seed=11)
+    p = net.params().astype(np.float64)
+    assert len(p) == Morc.nparams(widths)
+    for n in (1, 37, 300):
+        X, T, Mk = batch(widths, n, n)
+        Y = net.forward(X)
+        assert close(Y, Morc.forward(widths, p, X)[-1]), (widths, n)
+        L = net.loss_grad(X, T, Mk)
+        Lo, go = Morc.loss_grad(widths, p, X.astype(np.float64), T.astype(np.float64),
+                                Mk.astype(np.float64))
+        assert close(L, Lo), (widths, n, L, Lo)
+        assert close(net.grads(), go), (widths, n)
+    net.free()
+
+
+def test_initialisation_is_he_uniform_and_seeded(B, gpu):
+    a = B.Mlp(WIDTHS, seed=5).params()
+    b = B.Mlp(WIDTHS, seed=5).params()
+    c = B.Mlp(WIDTHS, seed=6).params()
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    for W, bias in Morc.split(WIDTHS, a):
+        lim = np.sqrt(6.0 / W.shape[1])
+        assert np.abs(W).max() <= lim and np.abs(W).max() > 0.8 * lim
+        assert not bias.any()
+
+
+def test_adam_steps_match_oracle(B, gpu):
+    net = B.Mlp(WIDTHS, seed=3)
+    p = net.params().astype(np.float64)
+    opt = Morc.Adam(len(p))
+    for k in range(4):
+        X, T, Mk = batch(WIDTHS, 64, 100 + k)
+        net.loss_grad(X, T, Mk)
+        net.adam_step(1e-2)
+        _, g = Morc.loss_grad(WIDTHS, p, X.astype(np.float64), T.astype(np.float64),
+                              Mk.astype(np.float64))
+        p = opt.step(p, g, 1e-2)
+        assert close(net.params(), p, 1e-4), k
+
+
+def test_training_fits_and_is_deterministic(B, gpu):
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(2000, 24)).astype(np.float32)
+    A = rng.normal(size=(24, 10)) / 5
+    T = (np.tanh(X @ A) + 0.1 * (X[:, :10] ** 2)).astype(np.float32)
+    Mk = np.ones_like(T)
+    runs = []
+    for _ in range(2):
+        net = B.Mlp(WIDTHS, seed=1)
+        trace = net.train(X, T, Mk, epochs=40, batch=128, lr=3e-3, seed=2)
+        runs.append((trace, net.params()))
+        net.free()
+    trace = runs[0][0]
+    assert trace[-1] < 0.1 * trace[0], trace
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_selector_install_and_choice(B, gpu):
+    dims = (60, 3000, 2000)
+    t = B.Coo.generate(dims, 50000, seed=4)
+    dev = B.DeviceTensor(B.encode(t), value_dtype=1)
+    f = dev.features(1, 16, B.F32)
+    assert f.shape == (B.SEL_NFEAT,)
+    assert np.isclose(f[0], np.log2(3000)) and f[4] == 4 and f[5] == 3 and f[6] == 2
+    assert abs(f[8:16].sum() - 1) < 1e-5 and abs(f[18:22].sum() - 1) < 1e-5
+    assert f[23] == 0 and dev.features(0, 16, B.F32)[23] == 1
+    # a network with zero weights predicts its last-layer biases
+    widths = [B.SEL_NFEAT, 8, B.SEL_NCAND]
+    nparams = B.SEL_NFEAT * 8 + 8 + 8 * B.SEL_NCAND + B.SEL_NCAND
+    xm, xs = np.zeros(B.SEL_NFEAT), np.ones(B.SEL_NFEAT)
+
+    def install(pref):
+        p = np.zeros(nparams, np.float32)
+        p[-B.SEL_NCAND:] = 5.0
+        for rank, c in enumerate(pref):
+            p[nparams - B.SEL_NCAND + c] = rank
+        B.selector_install(widths, p, xm, xs)
+
+    try:
+        install([6, 2])  # lvl3 with 8 partitions, then atomic
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_LVL3, 8)
+        assert dev.select_kernel(1, 16, B.F32) == (B.K_ATOMIC, 1)  # lvl3 cannot hold 3000 rows
+        install([1])
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_BATCHED, 1)
+        B.selector_install(None, None, None, None)
+        assert dev.select_kernel(1, 16, B.F32) == (dev.auto_kernel(1, 16), 0)
+    finally:
+        B.selector_install(None, None, None, None)
+    dev.free()
+
+
+def test_timed_mttkrp_reports_the_kernel(B, gpu):
+    dims = (50, 400, 300)
+    t = B.Coo.generate(dims, 20000, seed=8)
+    dev = B.DeviceTensor(B.encode(t), value_dtype=1)
+    F = [B.DeviceArray.from_numpy(np.ones((d, 16)), B.F32) for d in dims]
+    out = B.DeviceArray((50, 16), B.F32)
+    for c in range(B.SEL_NCAND):
+        if not dev.candidate_valid(0, 16, B.F32, c):
+            continue
+        k, np_ = B.sel_candidate(c)
+        ms, used = B.mttkrp_time(dev, 0, 16, F, out, B.F32, k, np_, reps=2)
+        assert ms > 0 and used == k
+    dev.free()
