@@ -124,9 +124,10 @@ def main():
                  s_per_iter=r["iter_ms"] / max(r["iters"], 1) / 1e3,
                  mttkrp_gbs=sum(abytes) / (sum(per_mode_ms) * 1e-3) / 1e9)
         results[name] = r
-        log("%s: %.4f s/iter (host %.4f), MTTKRP ms/mode %s, GB/s %s, final fit %.9f" % (
+        log("%s: %.4f s/iter (host %.4f), MTTKRP ms/mode %s, GB/s %s, kernels %s, final fit %.9f" % (
             name, r["s_per_iter"], r["wall"] / max(args.steps, 1),
-            np.round(per_mode_ms, 3).tolist(), np.round(gbs, 1).tolist(), r["fits"][-1]))
+            np.round(per_mode_ms, 3).tolist(), np.round(gbs, 1).tolist(), r["kernels"].tolist(),
+            r["fits"][-1]))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -149,7 +150,9 @@ def main():
         "data": "synthetic (seeded counter-based Poisson generator)",
         "config": {"workload": args.workload, "dims": list(dims), "nnz": int(nnz), "rank": R,
                    "zipf": list(W["zipf"]) if W["zipf"] else None,
-                   "kernel_choice": "heuristic (-k 10 rule; learned selector not installed)",
+                   "kernel_choice": ("learned selector" if B.selector_installed()
+                                     else "-k 10 rule (no selector installed)"),
+                   "kernels_per_mode_fp32": main_r["kernels"].tolist(),
                    "parallelism": "single GPU"},
         "mttkrp_gbs_fp32": round(main_r["mttkrp_gbs"], 1),
         "mttkrp_ms_per_mode_fp32": [round(x, 4) for x in main_r["per_mode_ms"]],
@@ -158,8 +161,9 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(main_r["mttkrp_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "k_mttkrp_lvl1 (all modes), algorithmic bytes "
-                               "nnz*(8+4) + sum_m I_m*R*4 per launch"},
+                     "kernel": "MTTKRP launches of one FP32 iteration (kernel ids %s per mode); "
+                               "algorithmic bytes nnz*(8+4) + sum_m I_m*R*4 per launch"
+                               % main_r["kernels"].tolist()},
     }
     if "fp64" in results:
         r64 = results["fp64"]

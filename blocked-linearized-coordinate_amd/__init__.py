@@ -20,9 +20,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libblco.so")
 
 F64, F32, BF16 = 0, 1, 2
-K_LVL1, K_LVL2, K_LVL3, K_AUTO, K_BATCHED, K_BATCHED2, K_ATOMIC, K_SELECT = 1, 2, 3, 10, 11, 12, 13, 20
+K_LVL1, K_LVL2, K_LVL3, K_AUTO, K_BATCHED, K_BATCHED2, K_ATOMIC, K_SORTED, K_SELECT = \
+    1, 2, 3, 10, 11, 12, 13, 14, 20
 MAX_MODES = 5
-SEL_NFEAT, SEL_NCAND = 24, 10
+SEL_NFEAT, SEL_NCAND = 24, 11
 
 _lib = None
 

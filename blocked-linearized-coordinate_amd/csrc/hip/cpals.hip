@@ -24,43 +24,16 @@
 #include <vector>
 
 #include "als.hpp"
+#include "cpals.hpp"
 #include "device.hpp"
 #include "devcommon.hpp"
 #include "mttkrp.hpp"
 
 using namespace blco;
 
-struct blco_cpals_state {
-    blco_dev* d = nullptr;
-    blco_cpals_opts o{};
-    int N = 0, R = 0, ft = BLCO_F64, mt = BLCO_F64;
-    hipStream_t st = nullptr;
-    void* U[BLCO_MAX_MODES] = {};
-    double* G[BLCO_MAX_MODES] = {};
-    void* Mbuf = nullptr;
-    double *lambda = nullptr, *inv = nullptr, *fac = nullptr, *work = nullptr;
-    int* status = nullptr;
-    double *gram_part = nullptr, *stats = nullptr, *ip = nullptr, *fitbuf = nullptr;
-    double* norm_part = nullptr;
-    double* h_fit = nullptr;  // pinned
-    double normX2 = 0.0;
-    int order[BLCO_MAX_MODES] = {};
-    int kern[BLCO_MAX_MODES] = {};
-    hipEvent_t ev0[BLCO_MAX_MODES] = {}, ev1[BLCO_MAX_MODES] = {};
-    hipEvent_t it0 = nullptr, it1 = nullptr;
-    double mttkrp_ms[BLCO_MAX_MODES] = {};
-    double iter_ms = 0.0;
-    int iters_timed = 0;
-    int iters_done = 0;
-    double fit_old = 0.0;
-    bool stopped = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-};
-
 namespace blco {
 
-static void to_dtype(const double* src, void* dst, uint64_t n, int dt)
+void to_dtype(const double* src, void* dst, uint64_t n, int dt)
 {
     if (dt == BLCO_F64) std::memcpy(dst, src, n * 8);
     else if (dt == BLCO_F32) {
@@ -78,7 +51,7 @@ static void to_dtype(const double* src, void* dst, uint64_t n, int dt)
     }
 }
 
-static void from_dtype(const void* src, double* dst, uint64_t n, int dt)
+void from_dtype(const void* src, double* dst, uint64_t n, int dt)
 {
     if (dt == BLCO_F64) std::memcpy(dst, src, n * 8);
     else if (dt == BLCO_F32) {
@@ -95,32 +68,46 @@ static void from_dtype(const void* src, double* dst, uint64_t n, int dt)
     }
 }
 
+int enqueue_step(blco_cpals_state* s, int n, int which_norm, hipStream_t st)
+{
+    const int N = s->N, R = s->R;
+    const uint64_t I = s->d->dims[n];
+    GramSet gs;
+    for (int m = 0; m < N; ++m) gs.g[m] = s->G[m];
+    const int last = s->order[N - 1];
+    int rc = launch_factorize(gs, N, n, R, s->fac, s->work, s->status, st);
+    if (!rc) rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, st);
+    if (!rc) rc = launch_lambda(s->stats, (int)solve_parts(I), R, which_norm, s->lambda, s->inv, st);
+    if (!rc) rc = launch_gram(s->U[n], s->ft, I, R, s->inv, n == last ? s->Mbuf : nullptr, s->gram_part,
+                              s->G[n], s->ip, st);
+    return rc;
+}
+
+int enqueue_fit(blco_cpals_state* s, hipStream_t st)
+{
+    GramSet gs;
+    for (int m = 0; m < s->N; ++m) gs.g[m] = s->G[m];
+    return launch_fit(gs, s->N, s->R, s->lambda, s->ip, s->normX2, s->fitbuf, st);
+}
+
 // Enqueues one ALS sweep (all modes) and the fit kernel.
 static int enqueue_sweep(blco_cpals_state* s, int which_norm, bool timed)
 {
     blco_dev* d = s->d;
     const int N = s->N, R = s->R;
-    GramSet gs;
-    for (int m = 0; m < N; ++m) gs.g[m] = s->G[m];
-    const int last = s->order[N - 1];
     for (int k = 0; k < N; ++k) {
         const int n = s->order[k];
-        const uint64_t I = d->dims[n];
-        int rc = launch_fill(s->Mbuf, I * (uint64_t)R, s->mt, 0.0, s->st);
+        int rc = launch_fill(s->Mbuf, d->dims[n] * (uint64_t)R, s->mt, 0.0, s->st);
         if (rc) return rc;
         if (timed) HIP_TRY(hipEventRecord(s->ev0[n], s->st));
         rc = mttkrp(d, n, R, s->ft, s->U, s->Mbuf, s->o.kernel, s->o.nprtn, s->o.thread_cf, s->st,
                     &s->kern[n]);
         if (rc) return rc;
         if (timed) HIP_TRY(hipEventRecord(s->ev1[n], s->st));
-        rc = launch_factorize(gs, N, n, R, s->fac, s->work, s->status, s->st);
-        if (!rc) rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, s->st);
-        if (!rc) rc = launch_lambda(s->stats, (int)solve_parts(I), R, which_norm, s->lambda, s->inv, s->st);
-        if (!rc) rc = launch_gram(s->U[n], s->ft, I, R, s->inv, n == last ? s->Mbuf : nullptr,
-                                  s->gram_part, s->G[n], s->ip, s->st);
+        rc = enqueue_step(s, n, which_norm, s->st);
         if (rc) return rc;
     }
-    return launch_fit(gs, N, R, s->lambda, s->ip, s->normX2, s->fitbuf, s->st);
+    return enqueue_fit(s, s->st);
 }
 
 static double now_ms()

@@ -32,6 +32,7 @@ namespace blco {
 constexpr uint64_t LDS_OUT_MAX = 48 * 1024;
 constexpr int LVL3_MAX_MODE_LENGTH = 100;     // src/utils.hpp:13
 constexpr int DEFAULT_TPW = 4;                // tiles of 64 per wave
+constexpr int SORTED_TPW = 16;                // tiles of 64 per wave, row-ordered kernel
 constexpr int DEFAULT_NPRTN = 1;
 constexpr uint32_t LVL3_CHUNKS_PER_WG = 64;
 constexpr uint32_t HOT_EXCLUDE = 64;  // hot rows kept out of the row windows
@@ -159,6 +160,8 @@ const WindowTab* window_table(blco_dev* d, int k)
 constexpr uint64_t PART_MAX_BYTES = 4ull << 30;
 // An HBM atomic add of one row costs about this many plain row stores.
 constexpr double ATOMIC_ROW_COST = 32.0;
+// Partial rows summed by one reduce lane group before it adds to the output.
+constexpr uint64_t REDUCE_SEG = 16;
 
 // Per-entry use of the LDS window: it pays when storing the window's rows
 // (and reading them back in the reduce) costs less than the HBM atomics of
@@ -303,12 +306,19 @@ const WindowPlan* window_plan(blco_dev* d, int mode, int rank, int acc)
         for (uint64_t i = 0; i < P; ++i) order[i] = (uint32_t)i;
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return prow[x] < prow[y]; });
     }
+    // segments of at most REDUCE_SEG partial rows of one output row; a row
+    // split over several segments is marked (top bit) for an atomic add
     std::vector<uint64_t> rows, ptr;
-    for (uint64_t i = 0; i < P; ++i)
-        if (i == 0 || prow[order[i]] != prow[order[i - 1]]) {
-            rows.push_back(prow[order[i]]);
-            ptr.push_back(i);
+    for (uint64_t i = 0; i < P;) {
+        uint64_t j = i + 1;
+        while (j < P && prow[order[j]] == prow[order[i]]) ++j;
+        const uint64_t flag = j - i > REDUCE_SEG ? REDUCE_SHARED : 0;
+        for (uint64_t s0 = i; s0 < j; s0 += REDUCE_SEG) {
+            rows.push_back(prow[order[i]] | flag);
+            ptr.push_back(s0);
         }
+        i = j;
+    }
     ptr.push_back(P);
     p.nred = rows.size();
     auto up = [&](auto** dst, const auto& v) {
@@ -338,10 +348,12 @@ bool lvl3_output_fits(const blco_dev* d, int mode, int rank, int ftype)
 
 int auto_kernel(const blco_dev* d, int mode, int rank, int ftype)
 {
-    // src/alto_dev.cu:1975,1996-2001: short modes -> lvl3, else lvl1.
+    // src/alto_dev.cu:1975,1996-2001: short modes -> lvl3, else lvl1; here
+    // the long modes run the row-ordered kernel when the tensor is resident
+    // (the row window when it is streamed or the copy does not fit).
     if (d->dims[mode] <= (uint64_t)LVL3_MAX_MODE_LENGTH && lvl3_output_fits(d, mode, rank, ftype))
         return BLCO_K_LVL3;
-    return BLCO_K_BATCHED2;
+    return d->stream_data || d->sorted_failed[mode] ? BLCO_K_BATCHED2 : BLCO_K_SORTED;
 }
 
 static int ensure_partials(blco_dev* d, size_t bytes)
@@ -357,6 +369,40 @@ static int ensure_partials(blco_dev* d, size_t bytes)
 
 // One kernel launch over chunks [c0, c1) whose data lives at coords/vals
 // indexed from coord_base.
+// Builds the row-ordered copy of a mode (kernel 14) from the resident
+// tensor: download, stable sort of every block by the mode index on the
+// host, upload.  Once per mode.
+static int ensure_row_ordered(blco_dev* d, int mode)
+{
+    if (d->d_scoords[mode] || !d->nnz) return BLCO_OK;
+    const uint64_t n = d->nnz;
+    const size_t vsz = value_size(d->vdtype);
+    std::vector<uint64_t> coords(n), perm(n), sc(n);
+    std::vector<char> vals(n * vsz), sv(n * vsz);
+    HIP_TRY(hipMemcpy(coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vals.data(), d->d_vals, n * vsz, hipMemcpyDeviceToHost));
+    row_order(mode, d->masks, d->pos, n, coords.data(), d->nblocks, d->bstart.data(), d->bnnz.data(), perm.data());
+    gather_bytes(coords.data(), 8, perm.data(), n, sc.data());
+    gather_bytes(vals.data(), vsz, perm.data(), n, sv.data());
+    uint64_t* dc = nullptr;
+    void* dv = nullptr;
+    if (hipMalloc(&dc, n * 8) != hipSuccess || hipMalloc(&dv, n * vsz) != hipSuccess) {
+        if (dc) (void)hipFree(dc);
+        return fail(BLCO_ENOMEM, "no device memory for the row-ordered copy of mode %d (%llu bytes)", mode,
+                    (unsigned long long)(n * (8 + vsz)));
+    }
+    if (hipMemcpy(dc, sc.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dv, sv.data(), n * vsz, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dc);
+        (void)hipFree(dv);
+        return fail(BLCO_EHIP, "upload of the row-ordered copy failed");
+    }
+    d->d_scoords[mode] = dc;
+    d->d_svals[mode] = dv;
+    d->bytes += n * (8 + vsz);
+    return BLCO_OK;
+}
+
 static int launch_range(blco_dev* d, int which, MttkrpArgs a, const ChunkTab* t, uint32_t c0,
                         uint32_t c1, int ftype, hipStream_t st)
 {
@@ -392,7 +438,6 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     for (int m = 0; m < d->nmodes; ++m)
         if (m != mode && !factors[m]) return fail(BLCO_EINVAL, "factor %d is null", m);
     if (d->nmodes < 2) return fail(BLCO_EUNSUPPORTED, "MTTKRP needs at least 2 modes");
-    if (tpw <= 0) tpw = DEFAULT_TPW;
     if (tpw > 64) return fail(BLCO_EINVAL, "thread_cf %d > 64", tpw);
     if (nprtn <= 0) nprtn = DEFAULT_NPRTN;
     if (nprtn > 1024) return fail(BLCO_EINVAL, "nprtn %d > 1024", nprtn);
@@ -406,7 +451,8 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
             kernel = BLCO_K_AUTO;
         }
     }
-    if (kernel == BLCO_K_AUTO) kernel = auto_kernel(d, mode, rank, ftype);
+    const bool chosen = kernel == BLCO_K_AUTO;
+    if (chosen) kernel = auto_kernel(d, mode, rank, ftype);
     int which;
     switch (kernel) {
         case BLCO_K_LVL1:
@@ -423,10 +469,30 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
             }
             break;
         case BLCO_K_ATOMIC: which = W_ATOMIC; break;
+        case BLCO_K_SORTED:
+            which = W_SORTED;
+            if (d->stream_data) {  // the copies are device-resident only
+                which = W_WINDOW;
+                kernel = BLCO_K_BATCHED2;
+            }
+            break;
         default: return fail(BLCO_EINVAL, "unknown kernel id %d", kernel);
     }
+    if (tpw <= 0) tpw = which == W_SORTED ? SORTED_TPW : DEFAULT_TPW;
     if (used_kernel) *used_kernel = kernel;
     if (d->nnz == 0) return BLCO_OK;
+    if (which == W_SORTED) {
+        int rc = ensure_row_ordered(d, mode);
+        if (rc == BLCO_ENOMEM && chosen) {  // no room for the copy: the row window
+            d->sorted_failed[mode] = true;
+            which = W_WINDOW;
+            kernel = BLCO_K_BATCHED2;
+            if (used_kernel) *used_kernel = kernel;
+            tpw = DEFAULT_TPW;
+        } else if (rc) {
+            return rc;
+        }
+    }
     const ChunkTab* t = chunk_table(d, tpw);
     if (!t) return BLCO_ENOMEM;
 
@@ -464,6 +530,10 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         const size_t need = wp->prows * (size_t)rank * acc;
         if (d->part_bytes < need) {
             if (d->part) (void)hipFree(d->part);
+    for (int m = 0; m < BLCO_MAX_MODES; ++m) {
+        if (d->d_scoords[m]) (void)hipFree(d->d_scoords[m]);
+        if (d->d_svals[m]) (void)hipFree(d->d_svals[m]);
+    }
             d->part = nullptr;
             d->part_bytes = 0;
             HIP_TRY(hipMalloc(&d->part, need));
@@ -538,6 +608,14 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         if (!rc)
             rc = launch_window_reduce(out, d->part, wp->d_rows, wp->d_ptr, wp->d_idx, wp->nred, rank,
                                       otype, st);
+    } else if (which == W_SORTED) {
+        a.coords = d->d_scoords[mode];
+        a.vals = d->d_svals[mode];
+        a.coord_base = 0;
+        // 32-bit row offsets when every factor's rank-wide element offset fits
+        bool wide = false;
+        for (int m = 0; m < d->nmodes; ++m) wide |= d->dims[m] * (uint64_t)rank > 0xFFFFFFFFull;
+        rc = launch_range(d, wide ? W_SORTED64 : W_SORTED, a, t, 0, t->n, ftype, st);
     } else if (!d->stream_data) {
         a.coords = d->d_coords;
         a.vals = d->d_vals;

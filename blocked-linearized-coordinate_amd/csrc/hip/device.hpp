@@ -41,8 +41,8 @@ struct WindowPlan {
     uint64_t* d_poff = nullptr;  // [entry + 1]: window of entry e = partial rows [poff[e], poff[e+1])
     uint64_t hot_poff = 0;       // first partial row of the hot areas
     uint64_t prows = 0;          // partial rows in all
-    uint64_t nred = 0;           // output rows with at least one partial row
-    uint64_t* d_rows = nullptr;  // [nred] those rows
+    uint64_t nred = 0;           // reduce segments (<= REDUCE_SEG partial rows of one output row)
+    uint64_t* d_rows = nullptr;  // [nred] output row of each segment, | REDUCE_SHARED when split
     uint64_t* d_ptr = nullptr;   // [nred + 1] CSR into d_idx
     uint32_t* d_idx = nullptr;   // [prows] partial rows, grouped by output row
 };
@@ -91,6 +91,11 @@ struct blco_dev {
     std::vector<uint64_t> hot_cnt_h[BLCO_MAX_MODES];  // their nonzero counts
     uint32_t* d_hot_list[BLCO_MAX_MODES] = {};
     int32_t* d_hot_rank[BLCO_MAX_MODES] = {};
+    // row-ordered copies (kernel 14): BLCO words and values of every block
+    // sorted by the mode's index, built on first use
+    uint64_t* d_scoords[BLCO_MAX_MODES] = {};
+    void* d_svals[BLCO_MAX_MODES] = {};
+    bool sorted_failed[BLCO_MAX_MODES] = {};  // no device memory for the copy
     void* part = nullptr;  // partial rows of the row-window MTTKRP
     size_t part_bytes = 0;
     // selector features (rank- and type-independent entries) and decisions

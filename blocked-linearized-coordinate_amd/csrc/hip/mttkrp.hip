@@ -37,27 +37,32 @@
 
 namespace blco {
 
-// out[rows[u]][c] += sum over j in [ptr[u], ptr[u+1]) of part[idx[j]][c]:
-// one cell per lane, `rank` lanes per listed output row.
+// out[row(u)][c] += sum over j in [ptr[u], ptr[u+1]) of part[idx[j]][c]:
+// one lane per (segment, column); a segment holds at most a few partial rows,
+// so a hot output row split over many segments is summed in parallel and
+// added atomically.
 template <typename AT>
 __global__ __launch_bounds__(WG) void k_window_reduce(AT* __restrict__ out, const AT* __restrict__ part,
                                                       const uint64_t* __restrict__ rows,
                                                       const uint64_t* __restrict__ ptr,
                                                       const uint32_t* __restrict__ idx,
-                                                      uint64_t nrows, int R)
+                                                      uint64_t nseg, int R)
 {
-    const uint64_t cells = nrows * (uint64_t)R;
+    const uint64_t cells = nseg * (uint64_t)R;
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < cells; i += (uint64_t)gridDim.x * WG) {
         const uint64_t u = i / R, c = i % R;
         const uint64_t j1 = ptr[u + 1];
         AT s0 = 0, s1 = 0;
         uint64_t j = ptr[u];
         for (; j + 1 < j1; j += 2) {
-            s0 += part[(uint64_t)idx[j] * R + c];
-            s1 += part[(uint64_t)idx[j + 1] * R + c];
+            s0 += stream_load(part + (uint64_t)idx[j] * R + c);
+            s1 += stream_load(part + (uint64_t)idx[j + 1] * R + c);
         }
-        if (j < j1) s0 += part[(uint64_t)idx[j] * R + c];
-        out[rows[u] * R + c] += s0 + s1;
+        if (j < j1) s0 += stream_load(part + (uint64_t)idx[j] * R + c);
+        const uint64_t rr = rows[u];
+        AT* o = out + (rr & ~REDUCE_SHARED) * R + c;
+        if (rr & REDUCE_SHARED) atomic_add_global(o, s0 + s1);
+        else *o += s0 + s1;
     }
 }
 
@@ -100,21 +105,6 @@ __global__ __launch_bounds__(WG) void k_fill(T* __restrict__ x, uint64_t n, T v)
 }
 
 // -------------------------------------------------------------- dispatch
-
-static Shape pick_shape(int rank, int esize)
-{
-    switch (rank) {
-        case 8: return {1, 8, true};
-        case 16: return {2, 8, true};
-        case 32: return esize == 8 ? Shape{2, 16, true} : Shape{4, 8, true};
-        case 64: return esize == 8 ? Shape{2, 32, true} : esize == 4 ? Shape{4, 16, true} : Shape{8, 8, true};
-        default: break;
-    }
-    int L = 8;
-    while (L < rank && L < WAVE) L <<= 1;
-    return {1, L, false};
-}
-
 
 
 const void* mttkrp_kernel(int which, int nmodes, int ftype, int rank, int* C, int* L)

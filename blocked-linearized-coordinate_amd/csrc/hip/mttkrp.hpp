@@ -56,14 +56,18 @@ struct MttkrpArgs {
     void* part;                // partial rows (rank wide), reduced by k_window_reduce
 };
 
-enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4 };
+enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4, W_SORTED = 5, W_SORTED64 = 6 };
 
 constexpr uint32_t FINE_CHUNK = 1024;        // nonzeros per range-table entry
 constexpr uint32_t WINDOW_LDS_BYTES = 40 * 1024;  // hot area + row window per workgroup
 constexpr int WIN_WAVES = 8;                      // waves per row-window workgroup
 constexpr int WIN_BATCH = 4;                      // positions whose gathers are issued together
 
-// out[rows[u]][:] += sum of the partial rows idx[ptr[u] .. ptr[u+1]).
+// Marks a reduce segment whose output row is split over several segments.
+constexpr uint64_t REDUCE_SHARED = 1ull << 63;
+
+// out[rows[u]][:] += sum of the partial rows idx[ptr[u] .. ptr[u+1]), with
+// an atomic add when rows[u] carries REDUCE_SHARED.
 int launch_window_reduce(void* out, const void* part, const uint64_t* rows, const uint64_t* ptr,
                          const uint32_t* idx, uint64_t nrows, int rank, int otype, hipStream_t st);
 

@@ -31,6 +31,14 @@ __device__ __forceinline__ void load_row(const FT* __restrict__ p,
     }
 }
 
+// Tensor coordinates and values are read once per MTTKRP: non-temporal
+// loads keep them from evicting the factor rows that the gathers reuse.
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
 template <typename AT>
 __device__ __forceinline__ void atomic_add_global(AT* p, AT v)
 {
@@ -78,8 +86,8 @@ __device__ __forceinline__ void stage_tile(const MttkrpArgs& a, const uint64_t (
 {
     const bool valid = (uint32_t)lane < nvalid;
     const VT* vals = reinterpret_cast<const VT*>(a.vals);
-    uint64_t coord = valid ? a.coords[e0 + lane - a.coord_base] : 0;
-    AT v = valid ? (AT)vals[e0 + lane - a.coord_base] : (AT)0;
+    uint64_t coord = valid ? stream_load(a.coords + (e0 + lane - a.coord_base)) : 0;
+    AT v = valid ? (AT)stream_load(vals + (e0 + lane - a.coord_base)) : (AT)0;
     uint64_t idx[N];
     uint64_t orow = 0;
     #pragma unroll
@@ -279,13 +287,10 @@ __global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a
     const VT* vals = reinterpret_cast<const VT*>(a.vals);
     const uint32_t ntiles = (sc.len + WAVE - 1) / WAVE;
 
-    auto flush = [&](uint64_t row, int col, const AT (&acc)[C]) {
-        const int32_t hr = a.hot_rank ? a.hot_rank[row] : -1;
-        AT* o;
-        bool lds = true;
-        if (hr >= 0 && (uint32_t)hr < a.hot_rows) o = s_hot + (uint64_t)hr * R + col;
-        else if (row >= lo && row - lo < wrows) o = s_win + (row - lo) * (uint64_t)R + col;
-        else { o = out + row * (uint64_t)R + col; lds = false; }
+    // dest: LDS row (hot area rows first, then the window) or -1 for HBM
+    auto flush = [&](uint64_t row, int32_t dest, int col, const AT (&acc)[C]) {
+        const bool lds = dest >= 0;
+        AT* o = lds ? s_hot + (uint64_t)dest * R + col : out + row * (uint64_t)R + col;
         #pragma unroll
         for (int cc = 0; cc < C; ++cc) {
             if (!(EXACT || col + cc < R)) continue;
@@ -300,8 +305,8 @@ __global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a
         // decode and sort the tile by output row, all in registers
         const bool valid = (uint32_t)lane < nvalid;
         const uint64_t el = sc.begin + t0 + lane - a.coord_base;
-        const uint64_t coord = valid ? a.coords[el] : 0;
-        const AT v = valid ? (AT)vals[el] : (AT)0;
+        const uint64_t coord = valid ? stream_load(a.coords + el) : 0;
+        const AT v = valid ? (AT)stream_load(vals + el) : (AT)0;
         IT idx[N];
         IT orow = 0;
         #pragma unroll
@@ -327,6 +332,14 @@ __global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a
         #pragma unroll
         for (int m = 0; m < N; ++m)
             if (m == mode) srow = sidx[m];
+        // each lane classifies its (sorted) nonzero once: one independent load
+        // instead of one dependent load per flushed segment
+        int32_t sdest = -1;
+        if ((uint32_t)lane < nvalid) {
+            const int32_t hr = a.hot_rank ? a.hot_rank[srow] : -1;
+            if (hr >= 0 && (uint32_t)hr < a.hot_rows) sdest = hr;
+            else if ((uint64_t)srow >= lo && (uint64_t)srow - lo < wrows) sdest = (int32_t)(a.hot_rows + ((uint64_t)srow - lo));
+        }
         // segment walk over positions [slot*L, slot*L + L)
         int kmax = (int)nvalid - slot * L;
         kmax = kmax < 0 ? 0 : (kmax > L ? L : kmax);
@@ -336,15 +349,18 @@ __global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a
             #pragma unroll
             for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
             uint64_t cur = 0;
+            int32_t cur_dest = -1;
             bool have = false;
             #pragma unroll
             for (int k0 = 0; k0 < L; k0 += PB) {
                 IT rr[PB];
+                int32_t dd[PB];
                 AT pv[PB][C];
                 #pragma unroll
                 for (int b = 0; b < PB; ++b) {
                     const int q = slot * L + k0 + b;
                     rr[b] = __shfl(srow, q);
+                    dd[b] = __shfl(sdest, q);
                     const AT vq = __shfl(sv, q);
                     #pragma unroll
                     for (int cc = 0; cc < C; ++cc) pv[b][cc] = vq;
@@ -373,7 +389,141 @@ __global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a
                 for (int b = 0; b < PB; ++b) {
                     if (k0 + b >= kmax) break;
                     if (have && (uint64_t)rr[b] != cur) {
-                        flush(cur, col, acc);
+                        flush(cur, cur_dest, col, acc);
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
+                    }
+                    cur = rr[b];
+                    cur_dest = dd[b];
+                    have = true;
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) acc[cc] += pv[b][cc];
+                }
+            }
+            if (have) flush(cur, cur_dest, col, acc);
+        }
+    }
+    __syncthreads();
+    AT* part = reinterpret_cast<AT*>(a.part);
+    const uint64_t hcells = (uint64_t)a.hot_rows * R;
+    // partial rows are read once, by the reduce: stored non-temporally
+    if (hcells) {
+        AT* hp = part + (a.hot_poff + (uint64_t)e * a.hot_rows) * R;
+        for (uint64_t i = threadIdx.x; i < hcells; i += WIN_WAVES * WAVE) __builtin_nontemporal_store(s_hot[i], hp + i);
+    }
+    if (wrows) {
+        AT* wp = part + a.sc_poff[e] * R;
+        for (uint64_t i = threadIdx.x; i < wrows * R; i += WIN_WAVES * WAVE) __builtin_nontemporal_store(s_win[i], wp + i);
+    }
+}
+
+// ------------------------------------------------------------ row-ordered
+
+// MTTKRP over a row-ordered copy (every block's nonzeros sorted by the output
+// index): a wave takes a chunk, each of its S = 64 / L slots a contiguous run
+// of the chunk.  Per round the L lanes of a slot decode L consecutive
+// nonzeros, one each, into the output row and the element offsets of the
+// input factor rows (IT: 32-bit when every index and offset fits); the slot
+// then walks them in order through cross-lane permutes, gathering WIN_BATCH
+// positions' factor rows at a time and keeping a register accumulator while
+// the output row repeats.  The next round's BLCO words and values are loaded
+// before this round's gathers, so the two latencies overlap.  Output rows are
+// contiguous, so a slot adds into HBM (atomically: a row may continue in the
+// next slot or chunk) about once per row it touches; the rows that end the
+// slots' runs are first combined across the slots of the wave.
+template <int N, typename FT, typename VT, int C, int L, bool EXACT, typename IT>
+__global__ __launch_bounds__(WG) void k_mttkrp_sorted(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    constexpr int S = WAVE / L;
+    constexpr int PB = L < WIN_BATCH ? L : WIN_BATCH;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = a.chunk0 + xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
+    if (c >= a.chunk0 + a.nchunks) return;  // whole wave leaves; no block barriers below
+    const Chunk ch = a.chunks[c];
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    const int R = a.rank, mode = a.mode;
+    const uint32_t run = (ch.len + S - 1) / S;
+    const uint32_t p0 = min((uint32_t)slot * run, ch.len), p1 = min(p0 + run, ch.len);
+    const uint32_t rounds = (run + L - 1) / L;  // the same for every slot
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+        const int col = cb + sub * C;
+        AT acc[C];
+        #pragma unroll
+        for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
+        uint64_t cur = 0;
+        bool have = false;
+        auto flush = [&]() {
+            AT* o = out + cur * (uint64_t)R + col;
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+                if (EXACT || col + cc < R) atomic_add_global(o + cc, acc[cc]);
+        };
+        auto fetch = [&](uint32_t r, uint64_t& coord, AT& v) {
+            const uint32_t q = p0 + r * L + sub;
+            const bool ok = r < rounds && q < p1;
+            const uint64_t el = ch.begin + q - a.coord_base;
+            coord = ok ? stream_load(a.coords + el) : 0;
+            v = ok ? (AT)stream_load(vals + el) : (AT)0;
+        };
+        uint64_t coord;
+        AT v;
+        fetch(0, coord, v);
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint32_t base = p0 + r * L;
+            // decode this round's nonzero: output row, factor-row offsets
+            IT row = 0, off[N];
+            #pragma unroll
+            for (int m = 0; m < N; ++m) {
+                const uint64_t i = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
+                off[m] = (IT)(i * (uint64_t)R);
+                if (m == mode) row = (IT)i;
+            }
+            const AT vr = v;
+            fetch(r + 1, coord, v);  // in flight during this round's gathers
+            const int nb = base < p1 ? (int)min((uint32_t)L, p1 - base) : 0;
+            #pragma unroll
+            for (int k0 = 0; k0 < L; k0 += PB) {
+                IT rr[PB];
+                AT pv[PB][C];
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    const int src = slot * L + k0 + b;
+                    rr[b] = __shfl(row, src);
+                    const AT vq = __shfl(vr, src);
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) pv[b][cc] = vq;
+                    #pragma unroll
+                    for (int m = 0; m < N; ++m) {
+                        if (m == mode) continue;
+                        const IT om = __shfl(off[m], src);
+                        AT x[C];
+                        if (k0 + b < nb) {
+                            const FT* f = reinterpret_cast<const FT*>(a.f[m]) + (uint64_t)om + col;
+                            if (EXACT || col + C <= R) {
+                                load_row<FT, C>(f, x);
+                            } else {
+                                #pragma unroll
+                                for (int cc = 0; cc < C; ++cc) x[cc] = (col + cc < R) ? (AT)to_acc(f[cc]) : (AT)0;
+                            }
+                        } else {
+                            #pragma unroll
+                            for (int cc = 0; cc < C; ++cc) x[cc] = 0;
+                        }
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) pv[b][cc] *= x[cc];
+                    }
+                }
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    if (k0 + b >= nb) break;
+                    if (have && (uint64_t)rr[b] != cur) {
+                        flush();
                         #pragma unroll
                         for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
                     }
@@ -383,19 +533,34 @@ __global__ __launch_bounds__(WIN_WAVES * WAVE) void k_mttkrp_window(MttkrpArgs a
                     for (int cc = 0; cc < C; ++cc) acc[cc] += pv[b][cc];
                 }
             }
-            if (have) flush(cur, col, acc);
         }
-    }
-    __syncthreads();
-    AT* part = reinterpret_cast<AT*>(a.part);
-    const uint64_t hcells = (uint64_t)a.hot_rows * R;
-    if (hcells) {
-        AT* hp = part + (a.hot_poff + (uint64_t)e * a.hot_rows) * R;
-        for (uint64_t i = threadIdx.x; i < hcells; i += WIN_WAVES * WAVE) hp[i] = s_hot[i];
-    }
-    if (wrows) {
-        AT* wp = part + a.sc_poff[e] * R;
-        for (uint64_t i = threadIdx.x; i < wrows * R; i += WIN_WAVES * WAVE) wp[i] = s_win[i];
+        // The slots' last segments: a row that runs over several slots (a
+        // hot row spans whole chunks) is summed across them first, with an
+        // inclusive segmented scan over the slots, and added by the slot
+        // that ends it: one HBM atomic per column per wave for that row.
+        const int below = lane - L, above = lane + L;
+        const uint64_t prev_cur = __shfl(cur, below < 0 ? lane : below);
+        const bool prev_have = __shfl(have, below < 0 ? lane : below);
+        bool head = slot == 0 || !have || !prev_have || prev_cur != cur;
+        #pragma unroll
+        for (int d = 1; d < S; d <<= 1) {
+            const int from = lane - d * L < 0 ? lane : lane - d * L;
+            const bool pf = __shfl(head, from);
+            AT pa[C];
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc) pa[cc] = __shfl(acc[cc], from);
+            if (slot >= d) {
+                if (!head) {
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) acc[cc] += pa[cc];
+                }
+                head = head || pf;
+            }
+        }
+        const uint64_t next_cur = __shfl(cur, above >= WAVE ? lane : above);
+        const bool next_have = __shfl(have, above >= WAVE ? lane : above);
+        const bool tail = slot == S - 1 || !next_have || next_cur != cur;
+        if (have && tail) flush();
     }
 }
 
@@ -421,8 +586,8 @@ __global__ __launch_bounds__(WG) void k_mttkrp_atomic(MttkrpArgs a)
         const uint32_t nvalid = min((uint32_t)WAVE, ch.len - t0);
         const bool valid = (uint32_t)lane < nvalid;
         const uint64_t e = ch.begin + t0 + lane - a.coord_base;
-        const uint64_t coord = valid ? a.coords[e] : 0;
-        const AT v = valid ? (AT)vals[e] : (AT)0;
+        const uint64_t coord = valid ? stream_load(a.coords + e) : 0;
+        const AT v = valid ? (AT)stream_load(vals + e) : (AT)0;
         uint64_t idx[N];
         #pragma unroll
         for (int m = 0; m < N; ++m) idx[m] = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
@@ -476,6 +641,8 @@ inline const void* kernel_ptr(int which)
     if (which == W_LVL3) return (const void*)&k_mttkrp_lvl3<N, FT, VT, C, L, EX>;
     if (which == W_WINDOW) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint32_t>;
     if (which == W_WINDOW64) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint64_t>;
+    if (which == W_SORTED) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint32_t>;
+    if (which == W_SORTED64) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint64_t>;
     return (const void*)&k_mttkrp_atomic<N, FT, VT, C, L, EX>;
 }
 
@@ -513,6 +680,20 @@ inline const void* select_n(int N, const Shape& s, int which)
         case 5: return select_shape<5, FT, VT>(s, which);
         default: return nullptr;
     }
+}
+
+inline Shape pick_shape(int rank, int esize)
+{
+    switch (rank) {
+        case 8: return {1, 8, true};
+        case 16: return {2, 8, true};
+        case 32: return esize == 8 ? Shape{2, 16, true} : Shape{4, 8, true};
+        case 64: return esize == 8 ? Shape{2, 32, true} : esize == 4 ? Shape{4, 16, true} : Shape{8, 8, true};
+        default: break;
+    }
+    int L = 8;
+    while (L < rank && L < WAVE) L <<= 1;
+    return {1, L, false};
 }
 
 const void* mttkrp_kernel_f64(int N, const Shape& s, int which);

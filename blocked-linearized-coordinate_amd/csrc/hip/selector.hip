@@ -42,8 +42,8 @@ namespace {
 // candidate c -> (kernel id, lvl3 partition count)
 constexpr int CAND_KERNEL[BLCO_SEL_NCAND] = {BLCO_K_BATCHED2, BLCO_K_BATCHED, BLCO_K_ATOMIC, BLCO_K_LVL3,
                                              BLCO_K_LVL3,     BLCO_K_LVL3,    BLCO_K_LVL3,  BLCO_K_LVL3,
-                                             BLCO_K_LVL3,     BLCO_K_LVL3};
-constexpr int CAND_NPRTN[BLCO_SEL_NCAND] = {1, 1, 1, 1, 2, 4, 8, 16, 32, 64};
+                                             BLCO_K_LVL3,     BLCO_K_LVL3,    BLCO_K_SORTED};
+constexpr int CAND_NPRTN[BLCO_SEL_NCAND] = {1, 1, 1, 1, 2, 4, 8, 16, 32, 64, 1};
 
 struct Selector {
     std::mutex mu;
@@ -168,6 +168,7 @@ int mode_features(const blco_dev* d, int mode, int rank, int ftype, float* out)
 bool candidate_valid(const blco_dev* d, int mode, int rank, int ftype, int c)
 {
     if (CAND_KERNEL[c] == BLCO_K_LVL3) return lvl3_output_fits(d, mode, rank, ftype);
+    if (CAND_KERNEL[c] == BLCO_K_SORTED) return !d->stream_data && !d->sorted_failed[mode];
     return true;
 }
 

@@ -35,6 +35,12 @@ struct Layout {
 // Computes the ALTO / BLCO layout of a shape; returns BLCO_OK or an error.
 int make_layout(int nmodes, const uint64_t* dims, Layout* L);
 
+// Stable permutation ordering every block's nonzeros by their mode index
+// (roworder.cpp), and dst[i] = src[perm[i]] for esz-byte elements.
+void row_order(int mode, const uint64_t* masks, const int* pos, uint64_t nnz, const uint64_t* coords,
+               uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm);
+void gather_bytes(const void* src, size_t esz, const uint64_t* perm, uint64_t n, void* dst);
+
 // Kernel-selector features of every mode that depend on the tensor only
 // (entries 0-3, 5, 7-22 of each row of `stride` floats; features.cpp).
 void compute_static_features(int N, const uint64_t* dims, const uint64_t* masks, const int* pos, uint64_t nnz,

@@ -1,7 +1,7 @@
 """Trains the learned MTTKRP kernel selector on timings from collect.py and
 reports its regret.
 
-The network (24 features -> 64 -> 64 -> 10 predicted log2 times) is trained
+The network (24 features -> 64 -> 64 -> 11 predicted log2 times) is trained
 on the GPU by the library's own kernels (blco_mlp_train: forward, masked MSE,
 backward, Adam) on the training tensors (t < 400); the candidates that cannot
 run for a mode are masked out of the loss.  On the held-out tensors
@@ -107,7 +107,8 @@ def main():
         rep[name] = dict(learned=regret(T, choose(pred[sel], valid[sel])),
                          rule_k10=regret(T, D["rule"][sel]),
                          always_window=regret(T, np.zeros(sel.sum(), int)),
-                         always_lvl1=regret(T, np.ones(sel.sum(), int)))
+                         always_lvl1=regret(T, np.ones(sel.sum(), int)),
+                         always_row_ordered=regret(T, np.full(sel.sum(), 10)))
         resid = (pred[sel] - logt[sel])[valid[sel]]
         rep[name]["log2_time_rmse"] = float(np.sqrt(np.mean(resid ** 2)))
     B.write_selector(args.out, widths, params, mu, sd)

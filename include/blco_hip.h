@@ -30,12 +30,18 @@ enum blco_kernel {
                               folded by a reduce), one launch per block; takes
                               the place of the reference's lvl2 stash kernel */
     BLCO_K_LVL3 = 3,       /* LDS-privatised output + nprtn partial matrices;
-                              modes too long for LDS run kernel 11 */
+                              modes too long for LDS run kernel 12 */
     BLCO_K_AUTO = 10,      /* the reference rule: I_n <= 100 (and fits) -> 3,
-                              else 12 */
+                              else 14 (12 when streamed or when the copy
+                              does not fit in device memory) */
     BLCO_K_BATCHED = 11,   /* segmented lvl1, all blocks in one launch */
     BLCO_K_BATCHED2 = 12,  /* row window, all blocks in one launch */
     BLCO_K_ATOMIC = 13,    /* one HBM atomic per (nonzero, column) */
+    BLCO_K_SORTED = 14,    /* segmented reduction over a row-ordered copy of
+                              the tensor (each block's nonzeros sorted by the
+                              output index; built on first use per mode,
+                              nnz * (8 + value size) more bytes per mode);
+                              streamed tensors run kernel 12 */
     BLCO_K_SELECT = 20     /* per-mode choice by the installed selector
                               (the rule of 10 when none is installed) */
 };
@@ -184,6 +190,39 @@ int blco_cpals(blco_dev* d, const blco_cpals_opts* opts, double* const* U,
                double* lambda, double* fit_trace, int* iters,
                double* final_fit);
 
+/* ---- many small tensors decomposed together ----
+ * (no reference counterpart: the reference decomposes one tensor per run,
+ * src/main.cpp:322-390).  A batch refers to resident device tensors owned
+ * by the caller (freed after the batch).  Mode n of the batch is mode n of
+ * every tensor that has it.  Factor and output tables are indexed by batch
+ * position: factors[t * BLCO_MAX_MODES + m], outs[t]. */
+typedef struct blco_batch blco_batch;
+int blco_batch_create(int ntensors, blco_dev* const* devs, blco_batch** out);
+void blco_batch_free(blco_batch* b);
+/* Times the persistent MTTKRP of `mode` over every (nonzeros per chunk /
+ * 64, waves per workgroup, workgroups per CU) in {1,2,4,8}^3 (at most 32
+ * waves per CU) and keeps the fastest for (mode, rank, factor type).  The
+ * outputs are overwritten with accumulated results. */
+int blco_batch_tune(blco_batch* b, int mode, int rank, int factor_dtype,
+                    void* const* factors, void* const* outs, int* tpw,
+                    int* waves, int* wgs_per_cu, float* ms);
+/* One persistent launch: outs[t] += MTTKRP(mode) of every tensor t that has
+ * the mode.  A zero shape argument takes the tuned shape (else 4, 4, 4).
+ * Synchronous. */
+int blco_batch_mttkrp(blco_batch* b, int mode, int rank, int factor_dtype,
+                      void* const* factors, void* const* outs, int tpw,
+                      int waves, int wgs_per_cu);
+/* CP-ALS of every tensor (cpals_blco_dev semantics per tensor, modes
+ * 0..N-1): per mode one persistent MTTKRP for the whole batch, the ALS steps
+ * of all tensors on their own streams, and after iteration 0 the sweep
+ * replayed as one captured hipGraph.  U (host f64, U[t * BLCO_MAX_MODES + m],
+ * dims x rank) holds the initial factors and receives the results; lambda
+ * (t * rank + r), fits[t] and iters[t] (may be NULL) receive each tensor's
+ * result at its own stopping iteration.  *iter_ms: mean device time of one
+ * sweep of the whole batch after iteration 0. */
+int blco_batch_cpals(blco_batch* b, const blco_cpals_opts* opts, double* const* U,
+                     double* lambda, double* fits, int* iters, double* iter_ms);
+
 /* Event-timed MTTKRP: one untimed call (builds tables and scratch), then
  * `reps` calls timed with HIP events on the tensor's own stream; *ms is the
  * mean time of one call.  The output accumulates reps + 1 results.  The
@@ -227,7 +266,8 @@ int blco_mlp_train(blco_mlp* h, const float* X, const float* T, const float* M,
 
 /* ---- learned MTTKRP kernel selector (kernel id 20) ----
  * Candidates c = 0..BLCO_SEL_NCAND-1: 0 row window (12), 1 segmented lvl1
- * (11), 2 atomic (13), 3..9 lvl3 with 1, 2, 4, 8, 16, 32, 64 partitions.
+ * (11), 2 atomic (13), 3..9 lvl3 with 1, 2, 4, 8, 16, 32, 64 partitions,
+ * 10 row-ordered (14).
  * The network maps BLCO_SEL_NFEAT features (normalised by the installed mean
  * and scale) to predicted log2 times of every candidate; the smallest
  * prediction among the candidates that can run is taken.  With no network
@@ -235,7 +275,7 @@ int blco_mlp_train(blco_mlp* h, const float* X, const float* T, const float* M,
  * (lib/../selector/selector.bin, or $BLCO_SELECTOR) is installed on first
  * use. */
 #define BLCO_SEL_NFEAT 24
-#define BLCO_SEL_NCAND 10
+#define BLCO_SEL_NCAND 11
 int blco_sel_candidate(int c, int* kernel, int* nprtn);
 /* 1 when candidate c can run for (mode, rank, factor type), else 0. */
 int blco_sel_valid(const blco_dev* d, int mode, int rank, int factor_dtype, int c);

@@ -73,7 +73,7 @@ def test_lvl1_f64_ranks(B, O, gpu, tensors, rank):
     check(B, O, tensors, "c1shape", 0, B.K_BATCHED, rank)
 
 
-@pytest.mark.parametrize("kernel", ["K_LVL1", "K_LVL2", "K_BATCHED2", "K_ATOMIC", "K_AUTO"])
+@pytest.mark.parametrize("kernel", ["K_LVL1", "K_LVL2", "K_BATCHED2", "K_ATOMIC", "K_SORTED", "K_AUTO"])
 def test_kernels_f64(B, O, gpu, tensors, kernel):
     check(B, O, tensors, "manyblocks", 0, getattr(B, kernel), 16)
     check(B, O, tensors, "c2shape", 0, getattr(B, kernel), 32)
@@ -84,6 +84,16 @@ def test_kernels_f64(B, O, gpu, tensors, kernel):
 def test_precision_forms(B, O, gpu, tensors, dtype, rank):
     check(B, O, tensors, "c2shape", dtype, B.K_BATCHED, rank)
     check(B, O, tensors, "order4", dtype, B.K_AUTO, rank)
+    check(B, O, tensors, "c2shape", dtype, B.K_SORTED, rank)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_row_ordered_all_shapes(B, O, gpu, tensors, name):
+    """Kernel 14 on every shape (multi-block and 65-bit included), f64 and
+    f32, and ranks that do not fill a lane group."""
+    check(B, O, tensors, name, 0, B.K_SORTED, 32)
+    check(B, O, tensors, name, 1, B.K_SORTED, 20)
+    check(B, O, tensors, name, 0, B.K_SORTED, 3, tpw=1)
 
 
 @pytest.mark.parametrize("nprtn", [1, 4, 17])
@@ -111,7 +121,7 @@ def test_thread_coarsening(B, O, gpu, tensors, tpw):
     check(B, O, tensors, "c2shape", 0, B.K_BATCHED, 32, tpw=tpw)
 
 
-@pytest.mark.parametrize("kernel", ["K_LVL1", "K_BATCHED", "K_ATOMIC", "K_LVL3"])
+@pytest.mark.parametrize("kernel", ["K_LVL1", "K_BATCHED", "K_ATOMIC", "K_LVL3", "K_SORTED"])
 def test_streamed_blocks(B, O, gpu, tensors, kernel):
     check(B, O, tensors, "manyblocks", 0, getattr(B, kernel), 16, stream=True)
     check(B, O, tensors, "bits65", 1, getattr(B, kernel), 32, stream=True)
