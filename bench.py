@@ -35,6 +35,79 @@ WORKLOADS = {
 }
 
 
+# The batched set: 64 tensors from seed 1003 * 1000 + t, order 3 or 4
+# (50/50), mode lengths log-uniform in [100, 5e4], nonzeros log-uniform in
+# [1e4, 2e6], uniform indices, Poisson values of a rank-8 Gamma(1,1) model;
+# rank 16, FP32 factors with FP64 solve.
+BATCH_SET = dict(count=64, seed=1003 * 1000, rank=16)
+
+
+def batch_specs():
+    out = []
+    for t in range(BATCH_SET["count"]):
+        rng = np.random.default_rng(BATCH_SET["seed"] + t)
+        order = 3 if rng.random() < 0.5 else 4
+        dims = tuple(int(round(10 ** rng.uniform(2, np.log10(5e4)))) for _ in range(order))
+        nnz = int(round(10 ** rng.uniform(4, np.log10(2e6))))
+        out.append((dims, nnz, BATCH_SET["seed"] + t))
+    return out
+
+
+def run_batched(B, args):
+    """One batch CP-ALS (persistent MTTKRP per mode + captured sweep) against
+    one plain CP-ALS per tensor, same initial factors, same iteration count."""
+    R = BATCH_SET["rank"]
+    t0 = time.perf_counter()
+    tens = []
+    for dims, nnz, seed in batch_specs():
+        c = B.Coo.generate(dims, nnz, gen_rank=8, value_kind=0, seed=seed)
+        tens.append((c, B.encode(c), B.init_factors(seed, dims, R)))
+    total = sum(int(c.nnz) for c, _, _ in tens)
+    log("batched set: %d tensors, %d nonzeros, generated in %.1f s" % (len(tens), total, time.perf_counter() - t0))
+    devs = [B.DeviceTensor(h, value_dtype=1) for _, h, _ in tens]
+    iters = args.warmup + args.steps
+    # plain: one solver per tensor, eager launches, after the same warm-up
+    plain_ms = 0.0
+    plain_fits = []
+    for dev, (_, _, U0) in zip(devs, tens):
+        s = B.CpAls(dev, U0, R, precision=B.F32)
+        s.iterate(args.warmup)
+        s.reset_times()
+        B.synchronize()
+        fits = s.iterate(args.steps)
+        plain_ms += s.times()["iter_ms"] / max(s.times()["iters"], 1)
+        plain_fits.append(fits[-1])
+        s.free()
+    bt = B.Batch(devs)
+    t0 = time.perf_counter()
+    _, _, fits, its, ms = bt.cpals([u for _, _, u in tens], R, max_iters=iters, precision=B.F32)
+    wall = time.perf_counter() - t0
+    bt.free()
+    for d in devs:
+        d.free()
+    fits, plain_fits = np.asarray(fits), np.asarray(plain_fits)
+    for k in np.nonzero(~np.isfinite(fits) | ~np.isfinite(plain_fits))[0]:
+        log("tensor %d %s: batch fit %r (%d iterations), plain fit %r"
+            % (k, tens[k][0].dims, fits[k], its[k], plain_fits[k]))
+    dfit = float(np.max(np.abs(fits - plain_fits)))
+    log("batched: %.3f ms per sweep of all tensors (plain per-tensor sum %.3f ms), max fit diff %.2e"
+        % (ms, plain_ms, dfit))
+    out = {
+        "metric": "CP-ALS seconds per iteration over the batched set (all tensors), 1 GPU",
+        "value": round(ms / 1e3, 6), "unit": "s/iter", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 factors/values, f64 solve/lambda/fit",
+        "data": "synthetic (seeded counter-based Poisson generator)",
+        "config": {"workload": "batched-64", "tensors": len(tens), "nnz_total": total, "rank": R,
+                   "parallelism": "single GPU"},
+        "plain_s_per_iter": round(plain_ms / 1e3, 6),
+        "speedup_vs_plain": round(plain_ms / ms, 3) if ms > 0 else None,
+        "max_fit_diff_vs_plain": dfit,
+        "host_wall_s": round(wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -80,10 +153,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nell2-shape-77M", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="nell2-shape-77M", choices=sorted(WORKLOADS) + ["batched-64"])
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device-to-device copy of BYTES (the reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -95,6 +171,9 @@ def main():
     if B.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batched-64":
+        run_batched(B, args)
+        return
     W = WORKLOADS[args.workload]
     dims, R = W["dims"], W["rank"]
 
@@ -107,6 +186,14 @@ def main():
     log("BLCO encode %.1f s, %d block(s)" % (time.perf_counter() - t0, h.nblocks))
     U0 = B.init_factors(W["seed"], dims, R)
     nnz = coo.nnz
+
+    if args.copy_calibration:
+        src = B.DeviceArray((args.copy_calibration // 8,), B.F64)
+        dst = B.DeviceArray((args.copy_calibration // 8,), B.F64)
+        B._check(B.lib().blco_memcpy_dtod(dst.ptr, src.ptr, src.nbytes))
+        log("copy calibration: %d bytes device to device" % src.nbytes)
+        src.free()
+        dst.free()
 
     results = {}
     forms = [("fp32", B.F32, 1)] + ([] if args.no_fp64 else [("fp64", B.F64, 0)])

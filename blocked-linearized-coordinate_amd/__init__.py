@@ -90,6 +90,7 @@ def lib():
             "blco_free": (C.c_int, [vp]),
             "blco_memcpy_htod": (C.c_int, [vp, vp, C.c_uint64]),
             "blco_memcpy_dtoh": (C.c_int, [vp, vp, C.c_uint64]),
+            "blco_memcpy_dtod": (C.c_int, [vp, vp, C.c_uint64]),
             "blco_memset_zero": (C.c_int, [vp, C.c_uint64]),
             "blco_synchronize": (C.c_int, []),
             "blco_dev_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
@@ -140,6 +141,15 @@ def lib():
             "blco_selector_installed": (C.c_int, []),
             "blco_select_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)]),
+            "blco_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+            "blco_batch_free": (None, [vp]),
+            "blco_batch_tune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_float)]),
+            "blco_batch_mttkrp": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                            C.c_int, C.c_int, C.c_int]),
+            "blco_batch_cpals": (C.c_int, [vp, C.POINTER(CpalsOpts), C.POINTER(vp), f64p, f64p, i32p,
+                                           C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -506,6 +516,65 @@ def write_selector(path, widths, params, xmean, xstd):
         f.write(np.asarray(xmean, np.float32).tobytes())
         f.write(np.asarray(xstd, np.float32).tobytes())
         f.write(np.asarray(params, np.float32).tobytes())
+
+
+class Batch:
+    """Many resident device tensors decomposed together (one persistent
+    MTTKRP kernel per mode, graph-captured ALS sweeps)."""
+
+    def __init__(self, devs):
+        self.devs = list(devs)
+        arr = (vp * len(self.devs))(*[vp(d.h) for d in self.devs])
+        h = vp()
+        _check(lib().blco_batch_create(len(self.devs), arr, C.byref(h)))
+        self.h = h.value
+
+    def _tables(self, factors, outs):
+        F = [0] * (len(self.devs) * MAX_MODES)
+        for t, fl in enumerate(factors):
+            for m, f in enumerate(fl):
+                F[t * MAX_MODES + m] = _dptr(f) or 0
+        return _ptr_array(F), _ptr_array([_dptr(o) or 0 for o in outs])
+
+    def tune(self, mode, rank, factors, outs, factor_dtype=F32):
+        """(tpw, waves, workgroups per CU, ms) picked for (mode, rank, type)."""
+        F, O = self._tables(factors, outs)
+        a, b, c, ms = C.c_int(0), C.c_int(0), C.c_int(0), C.c_float(0)
+        _check(lib().blco_batch_tune(self.h, mode, rank, factor_dtype, F, O, C.byref(a), C.byref(b),
+                                     C.byref(c), C.byref(ms)))
+        return a.value, b.value, c.value, ms.value
+
+    def mttkrp(self, mode, rank, factors, outs, factor_dtype=F32, tpw=0, waves=0, wgs_per_cu=0):
+        F, O = self._tables(factors, outs)
+        _check(lib().blco_batch_mttkrp(self.h, mode, rank, factor_dtype, F, O, tpw, waves, wgs_per_cu))
+
+    def cpals(self, U0s, rank, max_iters=20, tolerance=0.0, precision=F32):
+        """U0s[t][m] host f64 initial factors.  Returns (U, lambda, fits,
+        iters, ms per batch sweep after iteration 0)."""
+        T = len(self.devs)
+        o = _opts(rank, max_iters, tolerance, K_AUTO, precision, 0, 0, 0, False, False)
+        U = [[np.array(u, dtype=np.float64, order="C", copy=True) for u in ul] for ul in U0s]
+        ptrs = [0] * (T * MAX_MODES)
+        for t, ul in enumerate(U):
+            for m, u in enumerate(ul):
+                ptrs[t * MAX_MODES + m] = u.ctypes.data
+        lam = np.zeros(T * rank, np.float64)
+        fits = np.zeros(T, np.float64)
+        iters = np.zeros(T, np.int32)
+        ms = C.c_double(0)
+        _check(lib().blco_batch_cpals(self.h, C.byref(o), _ptr_array(ptrs), lam, fits, iters, C.byref(ms)))
+        return U, lam.reshape(T, rank), fits, iters, ms.value
+
+    def free(self):
+        if self.h:
+            lib().blco_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Mlp:

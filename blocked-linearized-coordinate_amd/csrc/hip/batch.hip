@@ -146,8 +146,8 @@ const void* batch_kernel(const Shape& s, size_t* per_wave)
         if (s.C == 1 && s.L == 8) BK(1, 8, true);
         if (s.C == 2 && s.L == 8) BK(2, 8, true);
         if constexpr (sizeof(FT) == 8) {
-            if (s.C == 2 && s.L == 16) BK(2, 16, true);
-            if (s.C == 2 && s.L == 32) BK(2, 32, true);
+            if (s.C == 4 && s.L == 8) BK(4, 8, true);
+            if (s.C == 4 && s.L == 16) BK(4, 16, true);
         } else if constexpr (sizeof(FT) == 4) {
             if (s.C == 4 && s.L == 8) BK(4, 8, true);
             if (s.C == 4 && s.L == 16) BK(4, 16, true);

@@ -400,7 +400,73 @@ static int ensure_row_ordered(blco_dev* d, int mode)
     d->d_scoords[mode] = dc;
     d->d_svals[mode] = dv;
     d->bytes += n * (8 + vsz);
+    // run starts of equal (block, row), for the row-aware chunk tables
+    std::vector<uint64_t>& runs = d->srun_h[mode];
+    runs.clear();
+    const uint64_t mk = d->masks[mode];
+    const int ps = d->pos[mode];
+    for (uint64_t b = 0; b < d->nblocks; ++b)
+        for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
+            if (i == d->bstart[b] || ((sc[i] >> ps) & mk) != ((sc[i - 1] >> ps) & mk)) runs.push_back(i);
+    runs.push_back(n);
     return BLCO_OK;
+}
+
+// Chunks of the row-ordered copy of a mode: within a block, a chunk closes at
+// the first row boundary once it holds 64 * tpw nonzeros, and never holds more
+// than twice that (a long row is cut), so most rows end inside a chunk while
+// every chunk stays short enough to spread a hot row over many waves.
+static const ChunkTab* sorted_chunk_table(blco_dev* d, int mode, int tpw)
+{
+    for (auto& t : d->stabs[mode])
+        if (t.tpw == tpw) return &t;
+    const uint64_t base = (uint64_t)WAVE * tpw, cap = 2 * base;
+    std::vector<Chunk> h;
+    const std::vector<uint64_t>& runs = d->srun_h[mode];
+    size_t r = 0;
+    for (uint64_t b = 0; b < d->nblocks; ++b) {
+        const uint64_t end = d->bstart[b] + d->bnnz[b];
+        uint64_t c0 = d->bstart[b];
+        while (r < runs.size() && runs[r] < d->bstart[b]) ++r;
+        auto emit = [&](uint64_t lo, uint64_t hi) {
+            h.push_back(Chunk{lo, (uint32_t)(hi - lo), (uint32_t)b});
+        };
+        // walk the runs of this block: [runs[r], runs[r+1]) clipped to the block
+        while (c0 < end) {
+            const uint64_t run_end = (r + 1 < runs.size()) ? std::min(runs[r + 1], end) : end;
+            if (run_end - c0 > cap) {  // inside a long row
+                emit(c0, c0 + cap);
+                c0 += cap;
+                continue;
+            }
+            if (run_end - c0 >= base || run_end == end) {
+                emit(c0, run_end);
+                c0 = run_end;
+            }
+            ++r;
+            if (r >= runs.size() || runs[r] >= end) {
+                if (c0 < end) emit(c0, end);
+                c0 = end;
+            }
+        }
+    }
+    ChunkTab t;
+    t.tpw = tpw;
+    t.n = (uint32_t)h.size();
+    if (t.n) {
+        if (hipMalloc(&t.d, sizeof(Chunk) * t.n) != hipSuccess) {
+            fail(BLCO_ENOMEM, "device allocation of the row-ordered chunk table failed");
+            return nullptr;
+        }
+        if (hipMemcpy(t.d, h.data(), sizeof(Chunk) * t.n, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(t.d);
+            fail(BLCO_EHIP, "upload of the row-ordered chunk table failed");
+            return nullptr;
+        }
+        d->bytes += sizeof(Chunk) * t.n;
+    }
+    d->stabs[mode].push_back(t);
+    return &d->stabs[mode].back();
 }
 
 static int launch_range(blco_dev* d, int which, MttkrpArgs a, const ChunkTab* t, uint32_t c0,
@@ -530,10 +596,6 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         const size_t need = wp->prows * (size_t)rank * acc;
         if (d->part_bytes < need) {
             if (d->part) (void)hipFree(d->part);
-    for (int m = 0; m < BLCO_MAX_MODES; ++m) {
-        if (d->d_scoords[m]) (void)hipFree(d->d_scoords[m]);
-        if (d->d_svals[m]) (void)hipFree(d->d_svals[m]);
-    }
             d->part = nullptr;
             d->part_bytes = 0;
             HIP_TRY(hipMalloc(&d->part, need));
@@ -612,10 +674,12 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         a.coords = d->d_scoords[mode];
         a.vals = d->d_svals[mode];
         a.coord_base = 0;
+        const ChunkTab* stab = sorted_chunk_table(d, mode, tpw);
+        if (!stab) return BLCO_ENOMEM;
         // 32-bit row offsets when every factor's rank-wide element offset fits
         bool wide = false;
         for (int m = 0; m < d->nmodes; ++m) wide |= d->dims[m] * (uint64_t)rank > 0xFFFFFFFFull;
-        rc = launch_range(d, wide ? W_SORTED64 : W_SORTED, a, t, 0, t->n, ftype, st);
+        rc = launch_range(d, wide ? W_SORTED64 : W_SORTED, a, stab, 0, stab->n, ftype, st);
     } else if (!d->stream_data) {
         a.coords = d->d_coords;
         a.vals = d->d_vals;
@@ -694,6 +758,16 @@ int blco_memcpy_dtoh(void* dst, const void* src, uint64_t bytes)
     if (!dst || !src) return fail(BLCO_EINVAL, "null pointer");
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BLCO_OK;
+}
+
+int blco_memcpy_dtod(void* dst, const void* src, uint64_t bytes)
+{
+    if (bytes == 0) return BLCO_OK;
+    if (!dst || !src) return fail(BLCO_EINVAL, "null pointer");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipDeviceSynchronize());
     return BLCO_OK;
 }
 
@@ -954,6 +1028,12 @@ void blco_dev_free(blco_dev* d)
         if (p.d_idx) (void)hipFree(p.d_idx);
     }
     if (d->part) (void)hipFree(d->part);
+    for (int m = 0; m < BLCO_MAX_MODES; ++m) {
+        if (d->d_scoords[m]) (void)hipFree(d->d_scoords[m]);
+        if (d->d_svals[m]) (void)hipFree(d->d_svals[m]);
+        for (auto& t : d->stabs[m])
+            if (t.d) (void)hipFree(t.d);
+    }
     if (d->d_bcoords) (void)hipFree(d->d_bcoords);
     if (d->d_coords) (void)hipFree(d->d_coords);
     if (d->d_vals) (void)hipFree(d->d_vals);

@@ -96,6 +96,9 @@ struct blco_dev {
     uint64_t* d_scoords[BLCO_MAX_MODES] = {};
     void* d_svals[BLCO_MAX_MODES] = {};
     bool sorted_failed[BLCO_MAX_MODES] = {};  // no device memory for the copy
+    // positions where a run of equal (block, row) starts in the copy, then nnz
+    std::vector<uint64_t> srun_h[BLCO_MAX_MODES];
+    std::list<blco::ChunkTab> stabs[BLCO_MAX_MODES];  // row-aware chunks of the copy, per tpw
     void* part = nullptr;  // partial rows of the row-window MTTKRP
     size_t part_bytes = 0;
     // selector features (rank- and type-independent entries) and decisions

@@ -72,13 +72,19 @@ int launch_window_reduce(void* out, const void* part, const uint64_t* rows, cons
     const uint64_t cells = nrows * (uint64_t)rank;
     if (!cells) return BLCO_OK;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((cells + WG - 1) / WG, 1u << 16);
+    hipError_t e = hipGetLastError();  // a launch before this one
+    if (e != hipSuccess)
+        return fail(BLCO_EHIP, "error pending before the window reduce: %s", hipGetErrorString(e));
     if (otype == BLCO_F64)
         hipLaunchKernelGGL(k_window_reduce<double>, dim3(grid), dim3(WG), 0, st, (double*)out,
                            (const double*)part, rows, ptr, idx, nrows, rank);
     else
         hipLaunchKernelGGL(k_window_reduce<float>, dim3(grid), dim3(WG), 0, st, (float*)out,
                            (const float*)part, rows, ptr, idx, nrows, rank);
-    HIP_TRY(hipGetLastError());
+    e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(BLCO_EHIP, "window reduce launch (grid %u, %llu segments, rank %d): %s", grid,
+                    (unsigned long long)nrows, rank, hipGetErrorString(e));
     return BLCO_OK;
 }
 
@@ -128,7 +134,11 @@ int launch_mttkrp(int which, const MttkrpArgs& a, int ftype, uint32_t grid, size
     const int block = (which == W_WINDOW || which == W_WINDOW64) ? WIN_WAVES * WAVE : WG;
     if (dyn_lds > 64 * 1024)
         HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds));
-    HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(block), params, dyn_lds, st));
+    hipError_t e = hipLaunchKernel(k, dim3(grid), dim3(block), params, dyn_lds, st);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(BLCO_EHIP, "MTTKRP launch (kind %d, grid %u x %d, %zu bytes LDS, rank %d): %s", which,
+                    grid, block, dyn_lds, a.rank, hipGetErrorString(e));
     return BLCO_OK;
 }
 

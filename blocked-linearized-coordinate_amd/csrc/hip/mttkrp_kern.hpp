@@ -15,7 +15,13 @@ __device__ __forceinline__ void load_row(const FT* __restrict__ p,
                                          typename Acc<FT>::type (&x)[C])
 {
     using AT = typename Acc<FT>::type;
-    if constexpr (C * sizeof(FT) == 16) {
+    if constexpr (C * sizeof(FT) == 32) {
+        const uint4* q = reinterpret_cast<const uint4*>(p);
+        uint4 u[2] = {q[0], q[1]};
+        const FT* e = reinterpret_cast<const FT*>(u);
+        #pragma unroll
+        for (int c = 0; c < C; ++c) x[c] = to_acc(e[c]);
+    } else if constexpr (C * sizeof(FT) == 16) {
         uint4 u = *reinterpret_cast<const uint4*>(p);
         const FT* e = reinterpret_cast<const FT*>(&u);
         #pragma unroll
@@ -653,8 +659,8 @@ inline const void* select_shape(const Shape& s, int which)
         if (s.C == 1 && s.L == 8) return kernel_ptr<N, FT, VT, 1, 8, true>(which);
         if (s.C == 2 && s.L == 8) return kernel_ptr<N, FT, VT, 2, 8, true>(which);
         if constexpr (sizeof(FT) == 8) {
-            if (s.C == 2 && s.L == 16) return kernel_ptr<N, FT, VT, 2, 16, true>(which);
-            if (s.C == 2 && s.L == 32) return kernel_ptr<N, FT, VT, 2, 32, true>(which);
+            if (s.C == 4 && s.L == 8) return kernel_ptr<N, FT, VT, 4, 8, true>(which);
+            if (s.C == 4 && s.L == 16) return kernel_ptr<N, FT, VT, 4, 16, true>(which);
         } else if constexpr (sizeof(FT) == 4) {
             if (s.C == 4 && s.L == 8) return kernel_ptr<N, FT, VT, 4, 8, true>(which);
             if (s.C == 4 && s.L == 16) return kernel_ptr<N, FT, VT, 4, 16, true>(which);
@@ -687,8 +693,8 @@ inline Shape pick_shape(int rank, int esize)
     switch (rank) {
         case 8: return {1, 8, true};
         case 16: return {2, 8, true};
-        case 32: return esize == 8 ? Shape{2, 16, true} : Shape{4, 8, true};
-        case 64: return esize == 8 ? Shape{2, 32, true} : esize == 4 ? Shape{4, 16, true} : Shape{8, 8, true};
+        case 32: return Shape{4, 8, true};
+        case 64: return esize == 2 ? Shape{8, 8, true} : Shape{4, 16, true};
         default: break;
     }
     int L = 8;

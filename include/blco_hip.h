@@ -56,6 +56,9 @@ int blco_malloc(void** ptr, uint64_t bytes);
 int blco_free(void* ptr);
 int blco_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 int blco_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy (the runtime's copy kernel; its HBM counters for a
+ * known size calibrate the traffic figures of the MTTKRP kernels). */
+int blco_memcpy_dtod(void* dst, const void* src, uint64_t bytes);
 int blco_memset_zero(void* ptr, uint64_t bytes);
 int blco_synchronize(void);
 

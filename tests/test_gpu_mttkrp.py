@@ -163,3 +163,24 @@ def test_bad_arguments_raise_before_launch(B, gpu, tensors):
         B.mttkrp(dev, 0, 16, F, out, factor_dtype=1)  # f32 factors on f64 values
     with pytest.raises(B.BlcoError):
         B.mttkrp(dev, 0, 16, [F[0], None, F[2]], out)  # missing factor
+
+
+def test_row_ordered_copy_survives_window_scratch_growth(B, O, gpu, tensors):
+    """Growing the row-window scratch releases the scratch only: the
+    row-ordered copies built before stay valid (checked after reusing
+    device memory)."""
+    t, h, at = tensors["c2shape"]
+    dev = B.DeviceTensor(h)
+    rng = np.random.default_rng(21)
+    U = [rng.random((int(d), 16)) for d in t.dims]
+    want = at.mttkrp(U, 0)
+    assert rel(run_mttkrp(B, dev, U, 0, 0, B.K_SORTED), want) <= 1e-12
+    for rank in (8, 16, 32):
+        Ur = [rng.random((int(d), rank)) for d in t.dims]
+        for m in range(t.nmodes):
+            run_mttkrp(B, dev, Ur, m, 0, B.K_BATCHED2)
+    junk = [B.DeviceArray.from_numpy(np.full(1 << 20, 7.0)) for _ in range(16)]
+    assert rel(run_mttkrp(B, dev, U, 0, 0, B.K_SORTED), want) <= 1e-12
+    for j in junk:
+        j.free()
+    dev.free()
