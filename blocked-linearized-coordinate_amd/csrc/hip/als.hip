@@ -146,15 +146,30 @@ __global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, 
         out[R * R + e] = ((s_ip[0][e] + s_ip[1][e]) + s_ip[2][e]) + s_ip[3][e];
 }
 
-// Sums nparts partial blocks of (R*R + R) in order: G (R*R) and ip (R).
+// Sums nparts partial blocks of (R*R + R): G (R*R) and ip (R).  A workgroup
+// takes 64 entries; its 4 waves sum the parts p = w, w + 4, ... of them and
+// the 4 sums are added in wave order (a fixed order: deterministic).
 __global__ __launch_bounds__(WG) void k_gram_reduce(const double* __restrict__ part, int nparts,
                                                     int R, double* __restrict__ G,
                                                     double* __restrict__ ip)
 {
+    __shared__ double s_sum[WAVES_PER_WG][WAVE];
     const int stride = R * R + R;
-    for (int e = blockIdx.x * WG + threadIdx.x; e < stride; e += gridDim.x * WG) {
-        double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * stride + e];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = blockIdx.x * WAVE + lane;
+    double s0 = 0.0, s1 = 0.0;
+    if (e < stride) {
+        int p = w;
+        for (; p + WAVES_PER_WG < nparts; p += 2 * WAVES_PER_WG) {
+            s0 += part[(uint64_t)p * stride + e];
+            s1 += part[(uint64_t)(p + WAVES_PER_WG) * stride + e];
+        }
+        if (p < nparts) s0 += part[(uint64_t)p * stride + e];
+    }
+    s_sum[w][lane] = s0 + s1;
+    __syncthreads();
+    if (w == 0 && e < stride) {
+        const double s = ((s_sum[0][lane] + s_sum[1][lane]) + s_sum[2][lane]) + s_sum[3][lane];
         if (e < R * R) G[e] = s;
         else if (ip) ip[e - R * R] = s;
     }
@@ -162,15 +177,19 @@ __global__ __launch_bounds__(WG) void k_gram_reduce(const double* __restrict__ p
 
 // ------------------------------------------------------------- factorize
 
-// status: 0 Cholesky in fac (lower, row-major), 1 pseudo-inverse in fac.
+// status: 0 V^{-1} in fac (by Cholesky), 1 pseudo-inverse in fac.
+// The Cholesky factor is formed right-looking (all trailing entries updated
+// in parallel after each column), which subtracts the terms of every entry
+// in the same order k = 0, 1, ... as dpotf2's column dot products.  L^{-1}
+// is formed row by row with the same parallel pattern, then
+// V^{-1} = L^{-T} L^{-1}.
 __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mode, int R,
                                                   double* __restrict__ fac,
                                                   double* __restrict__ work,
                                                   int* __restrict__ status)
 {
-    extern __shared__ double s_a[];  // R x R
-    __shared__ int s_fail;
-    __shared__ double s_piv;
+    extern __shared__ double s_a[];  // R x R: V, then L (lower); then R x R X = L^{-1}
+    double* s_x = s_a + R * R;
     const int tid = threadIdx.x;
     for (int e = tid; e < R * R; e += WG) {
         double v = 1.0;
@@ -178,54 +197,51 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
             if (m != mode) v *= gs.g[m][e];
         s_a[e] = v;
         work[e] = v;  // keep V for the fallback
+        s_x[e] = (e / R == e % R) ? 1.0 : 0.0;
     }
-    if (tid == 0) s_fail = 0;
     __syncthreads();
-    // in-place lower Cholesky, dpotf2 order (column j: pivot, then below)
+    bool ok = true;
     for (int j = 0; j < R; ++j) {
-        if (tid == 0) {
-            double ajj = s_a[j * R + j];
-            for (int k = 0; k < j; ++k) ajj -= s_a[j * R + k] * s_a[j * R + k];
-            if (!(ajj > 0.0)) s_fail = 1;
-            else s_piv = sqrt(ajj);
-            if (!s_fail) s_a[j * R + j] = s_piv;
+        const double ajj = s_a[j * R + j];
+        if (!(ajj > 0.0)) {  // every thread reads the same value: a uniform exit
+            ok = false;
+            break;
         }
+        const double piv = sqrt(ajj);
+        __syncthreads();  // all have read a_jj
+        for (int i = j + 1 + tid; i < R; i += WG) s_a[i * R + j] /= piv;
+        if (tid == 0) s_a[j * R + j] = piv;
         __syncthreads();
-        if (s_fail) break;
-        const double piv = s_piv;
-        for (int i = j + 1 + tid; i < R; i += WG) {
-            double s = s_a[i * R + j];
-            for (int k = 0; k < j; ++k) s -= s_a[i * R + k] * s_a[j * R + k];
-            s_a[i * R + j] = s / piv;
+        const int n = R - 1 - j;
+        for (int t = tid; t < n * n; t += WG) {
+            const int i = j + 1 + t / n, k = j + 1 + t % n;
+            if (k <= i) s_a[i * R + k] -= s_a[i * R + j] * s_a[k * R + j];
         }
         __syncthreads();
     }
-    if (!s_fail) {
-        // V^{-1} = L^{-T} L^{-1}: thread j forms column j of L^{-1} (forward
-        // substitution on e_j) into work, then every (a, b) entry is the dot
-        // of columns a and b of L^{-1} over k >= max(a, b).
-        double* Li = work;
-        for (int j = tid; j < R; j += WG) {
-            for (int i = 0; i < j; ++i) Li[i * R + j] = 0.0;
-            Li[j * R + j] = 1.0 / s_a[j * R + j];
-            for (int i = j + 1; i < R; ++i) {
-                double t = 0.0;
-                for (int k = j; k < i; ++k) t += s_a[i * R + k] * Li[k * R + j];
-                Li[i * R + j] = -t / s_a[i * R + i];
+    if (ok) {
+        for (int k = 0; k < R; ++k) {
+            const double lkk = s_a[k * R + k];
+            for (int c = tid; c <= k; c += WG) s_x[k * R + c] /= lkk;
+            __syncthreads();
+            const int n = R - 1 - k, w = k + 1;
+            for (int t = tid; t < n * w; t += WG) {
+                const int i = k + 1 + t / w, c = t % w;
+                s_x[i * R + c] -= s_a[i * R + k] * s_x[k * R + c];
             }
+            __syncthreads();
         }
-        __threadfence_block();
-        __syncthreads();
         for (int e = tid; e < R * R; e += WG) {
             int a = e / R, b = e % R;
             int k0 = a > b ? a : b;
             double t = 0.0;
-            for (int k = k0; k < R; ++k) t += Li[k * R + a] * Li[k * R + b];
+            for (int k = k0; k < R; ++k) t += s_x[k * R + a] * s_x[k * R + b];
             fac[e] = t;
         }
         if (tid == 0) *status = 0;
         return;
     }
+    __syncthreads();
     // ---- fallback: symmetric pseudo-inverse by cyclic Jacobi
     double* A = work;          // R x R, V (destroyed)
     double* Q = work + R * R;  // R x R eigenvectors (columns)
@@ -502,7 +518,7 @@ static int gram_t(UT* U, uint64_t rows, int R, const double* inv, const void* M,
 int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg)
 {
     uint64_t steps = (rows + 3) / 4;
-    uint64_t want = (steps + 15) / 16;  // ~16 row-steps per workgroup (4 per wave)
+    uint64_t want = (steps + 63) / 64;  // ~64 row-steps per workgroup (16 per wave)
     if (want < 1) want = 1;
     if (want > 512) want = 512;  // callers size partial buffers for 512 parts
     uint64_t rpw = (rows + want - 1) / want;
@@ -527,7 +543,7 @@ int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scal
     else rc = gram_t<bf16>((bf16*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
     if (rc) return rc;
     int stride = R * R + R;
-    hipLaunchKernelGGL(k_gram_reduce, dim3((stride + WG - 1) / WG), dim3(WG), 0, st, part, nparts, R, G, ip);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((stride + WAVE - 1) / WAVE), dim3(WG), 0, st, part, nparts, R, G, ip);
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }
@@ -535,7 +551,7 @@ int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scal
 int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double* fac, double* work,
                      int* status, hipStream_t st)
 {
-    size_t lds = sizeof(double) * (size_t)R * R;
+    size_t lds = sizeof(double) * 2 * (size_t)R * R;
     if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute((const void*)k_factorize,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_factorize, dim3(1), dim3(WG), lds, st, gs, nmodes, mode, R, fac, work, status);

@@ -139,3 +139,33 @@ def test_timed_mttkrp_reports_the_kernel(B, gpu):
         ms, used = B.mttkrp_time(dev, 0, 16, F, out, B.F32, k, np_, reps=2)
         assert ms > 0 and used == k
     dev.free()
+
+
+def test_selector_training_from_committed_timings(B, gpu):
+    """Trains the selector network with the library's kernels on the
+    committed timings of the 400 training tensors and checks the choice on
+    the 100 held-out ones: total-time regret against the exhaustive choice
+    <= 10%, and below the -k 10 rule's."""
+    import glob
+    sel = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "blocked-linearized-coordinate_amd", "selector", "data")
+    parts = [np.load(f) for f in sorted(glob.glob(os.path.join(sel, "*.npz")))]
+    D = {k: np.concatenate([p[k] for p in parts]) for k in ("feat", "times", "tensor", "rule")}
+    valid = ~np.isnan(D["times"])
+    logt = np.where(valid, np.log2(np.where(valid, D["times"], 1.0)), 0.0).astype(np.float32)
+    tr, te = D["tensor"] < 400, D["tensor"] >= 400
+    X = D["feat"].astype(np.float64)
+    mu, sd = X[tr].mean(axis=0), X[tr].std(axis=0)
+    sd = np.where(sd > 1e-6, sd, 1.0)
+    Xn = ((X - mu) / sd).astype(np.float32)
+    net = B.Mlp([B.SEL_NFEAT, 64, 64, B.SEL_NCAND], seed=7)
+    trace = net.train(Xn[tr], logt[tr], valid[tr].astype(np.float32), epochs=300, batch=128, lr=2e-3, seed=7)
+    assert trace[-1] < 0.05 * trace[0]
+    pred = net.forward(Xn[te])
+    net.free()
+    T = D["times"][te]
+    choice = np.argmin(np.where(valid[te], pred, np.inf), axis=1)
+    best = np.nanmin(T, axis=1)
+    learned = T[np.arange(len(T)), choice].sum() / best.sum() - 1
+    rule = T[np.arange(len(T)), D["rule"][te]].sum() / best.sum() - 1
+    assert learned <= 0.10 and learned < rule, (learned, rule)
