@@ -112,6 +112,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(workload, kernel_name):
+    """HBM bytes per launch of `kernel_name` for `workload` from the committed
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration it records), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "mttkrp_row_ordered_%s_fp32.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    in_traffic = False
+    for line in open(path):
+        if line.startswith("HBM traffic"):
+            in_traffic = True
+        elif in_traffic and kernel_name in line:
+            try:
+                return round(float(line.split()[-1]) * 1e6)  # total MB per dispatch
+            except ValueError:
+                return None
+    return None
+
+
 def algorithmic_bytes(dims, nnz, rank, s_v, s_f, mode):
     """Compulsory MTTKRP traffic of one mode: stream the tensor once (8-byte
     coordinate + value), read each input factor once, write the output once."""
@@ -247,7 +266,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(main_r["mttkrp_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(main_r["mttkrp_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": (profiled_traffic(args.workload, "k_mttkrp_sorted<3, float")
+                                 if list(main_r["kernels"]) == [14] * len(dims) else None),
+                     "traffic_source": "profiles/mttkrp_row_ordered_%s_fp32.txt (FETCH_SIZE x2.000 + "
+                                       "WRITE_SIZE x1.000, calibrated on a 1 GiB copy)" % args.workload,
                      "kernel": "MTTKRP launches of one FP32 iteration (kernel ids %s per mode); "
                                "algorithmic bytes nnz*(8+4) + sum_m I_m*R*4 per launch"
                                % main_r["kernels"].tolist()},
