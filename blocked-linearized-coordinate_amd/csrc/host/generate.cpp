@@ -83,8 +83,17 @@ extern "C" int blco_generate(const blco_gen_spec* sp, blco_coo** out)
     uint64_t ndraw = 0, unique = 0;
     uint64_t want = sp->target_nnz;
     int total_bits = L.total_bits;
+    // Each round's size follows the unique yield of the draws so far, so a
+    // skewed (Zipf) tensor with many duplicates converges in a few rounds.
+    // The result does not depend on the round sizes: draws are a prefix of
+    // one counter sequence and the first unique draws in draw order are kept.
+    double yield = 1.0;
     for (int round = 0; round < 64; ++round) {
-        uint64_t add = (round == 0) ? want : (uint64_t)((double)(want - unique) * 1.25) + 1024;
+        uint64_t add = want;
+        if (round > 0) {
+            yield = std::max(0.01, (double)unique / (double)ndraw);
+            add = (uint64_t)((double)(want - unique) / yield * 1.15) + 1024;
+        }
         uint64_t old = ndraw;
         ndraw += add;
         try {
