@@ -108,6 +108,73 @@ def run_batched(B, args):
     print(json.dumps(out), flush=True)
 
 
+# The large set: 4.8M x 1.8M x 1.8M (65 index bits: two BLCO key blocks),
+# 1.7e9 unique nonzeros, Zipf(1.2) on every mode (redrawn until unique),
+# log1p Poisson values of a rank-20 Gamma(1,1) model, rank 32, seed 1005.
+LARGE = dict(dims=(4_800_000, 1_800_000, 1_800_000), nnz=1_700_000_000, zipf=(1.2, 1.2, 1.2),
+             gen_rank=20, value_kind=1, rank=32, seed=1005)
+
+
+def run_large(B, args):
+    """FP32 + selector form and plain FP64 form, fully HBM-resident, then the
+    FP32 form with the tensor streamed from pinned host memory (device memory
+    held by the tensor: two slots of --slot-nnz nonzeros)."""
+    os.environ.setdefault("BLCO_PROGRESS", "1")
+    W, R = LARGE, LARGE["rank"]
+    nnz_req = int(W["nnz"] * args.large_scale)
+    t0 = time.perf_counter()
+    coo = B.Coo.generate(W["dims"], nnz_req, zipf=W["zipf"], gen_rank=W["gen_rank"],
+                         value_kind=W["value_kind"], seed=W["seed"], topup=True)
+    nnz = int(coo.nnz)
+    log("generated %d nonzeros in %.1f s" % (nnz, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    h = B.encode(coo)
+    del coo
+    log("BLCO encode %.1f s, %d block(s)" % (time.perf_counter() - t0, h.nblocks))
+    U0 = B.init_factors(W["seed"], W["dims"], R)
+    forms = [("fp32", B.F32, 1, False), ("fp64", B.F64, 0, False), ("fp32_streamed", B.F32, 1, True)]
+    if args.large_forms:
+        forms = [f for f in forms if f[0] in args.large_forms.split(",")]
+    res = {}
+    for name, prec, vdt, stream in forms:
+        t0 = time.perf_counter()
+        dev = B.DeviceTensor(h, value_dtype=vdt, stream_data=stream, slot_nnz=args.slot_nnz if stream else 0)
+        log("%s: device tensor in %.1f s, %.2f GB on the device" % (name, time.perf_counter() - t0,
+                                                                     dev.bytes() / 1e9))
+        kernel = B.K_SELECT if prec != B.F64 else B.K_AUTO
+        r = run_form(B, dev, U0, R, prec, kernel, args.warmup, args.steps)
+        dev_bytes = dev.bytes()
+        dev.free()
+        s_v = 8 if vdt == 0 else 4
+        s_f = 8 if prec == B.F64 else 4
+        per_mode_ms = r["mttkrp_ms"] / max(r["iters"], 1)
+        ab = [algorithmic_bytes(W["dims"], nnz, R, s_v, s_f, m) for m in range(3)]
+        res[name] = dict(s_per_iter=r["iter_ms"] / max(r["iters"], 1) / 1e3,
+                         mttkrp_ms_per_mode=[round(float(x), 3) for x in per_mode_ms],
+                         mttkrp_gbs=round(sum(ab) / (sum(per_mode_ms) * 1e-3) / 1e9, 1),
+                         kernels=r["kernels"].tolist(), fit=float(r["fits"][-1]),
+                         device_gb=round(dev_bytes / 1e9, 2))
+        log("%s: %.4f s/iter, MTTKRP ms/mode %s, kernels %s, fit %.9f" % (
+            name, res[name]["s_per_iter"], res[name]["mttkrp_ms_per_mode"], res[name]["kernels"],
+            res[name]["fit"]))
+    main = res.get("fp32") or next(iter(res.values()))
+    out = {
+        "metric": "CP-ALS s/iter, 1.7B-nonzero tensor, FP32+learned selector vs plain FP64, 1 GPU",
+        "value": round(main["s_per_iter"], 6), "unit": "s/iter", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(main["s_per_iter"] * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 factors/values, f64 solve/lambda/fit",
+        "data": "synthetic (seeded counter-based Poisson generator)",
+        "config": {"workload": "large-1.7B", "dims": list(W["dims"]), "nnz": nnz, "rank": R,
+                   "zipf": list(W["zipf"]), "blocks": int(h.nblocks), "parallelism": "single GPU"},
+        "forms": res,
+    }
+    if "fp32" in res and "fp64" in res:
+        out["speedup_fp32_vs_fp64"] = round(res["fp64"]["s_per_iter"] / res["fp32"]["s_per_iter"], 3)
+        out["fit_diff_fp32_vs_fp64"] = abs(res["fp32"]["fit"] - res["fp64"]["fit"])
+    print(json.dumps(out), flush=True)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -172,7 +239,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nell2-shape-77M", choices=sorted(WORKLOADS) + ["batched-64"])
+    ap.add_argument("--workload", default="nell2-shape-77M",
+                    choices=sorted(WORKLOADS) + ["batched-64", "large-1.7B"])
+    ap.add_argument("--large-scale", type=float, default=1.0,
+                    help="large-1.7B: fraction of the 1.7e9 nonzeros to generate")
+    ap.add_argument("--large-forms", default="", help="large-1.7B: comma list of fp32,fp64,fp32_streamed")
+    ap.add_argument("--slot-nnz", type=int, default=1 << 26,
+                    help="large-1.7B streamed form: nonzeros per device slot (two slots)")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -192,6 +265,9 @@ def main():
         sys.exit(1)
     if args.workload == "batched-64":
         run_batched(B, args)
+        return
+    if args.workload == "large-1.7B":
+        run_large(B, args)
         return
     W = WORKLOADS[args.workload]
     dims, R = W["dims"], W["rank"]

@@ -321,71 +321,80 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
 
 // ------------------------------------------------------------ solve rows
 
-// One thread per row: u = F m with F = V^{-1} (or V^+) from LDS, written as
-// the factor dtype; the same pass forms per-workgroup column statistics
-// (which 0: sum of squares, 1: max seeded with 1) into stats[blockIdx][R].
-template <typename MT, typename UT, int RR>
+// U = M F with F = V^{-1} (or V^+), row by row, written as the factor dtype,
+// and per-workgroup column statistics (which 0: sum of squares, 1: max
+// seeded with 1) into stats[blockIdx][R].  A group of G lanes (G = R rounded
+// up to a power of two) takes one row: lane j holds row j of F in registers
+// and forms u_j = sum_k F[j][k] m_k, with the row of M staged in LDS (a
+// coalesced load).  The workgroup walks rows with a grid stride.
+template <typename MT, typename UT, int G>
 __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT* __restrict__ U,
                                                    uint64_t rows, int R,
                                                    const double* __restrict__ fac, int which,
                                                    double* __restrict__ stats)
 {
-    extern __shared__ double s_f[];  // R x R, then WAVES_PER_WG x R stats
-    const int n = RR ? RR : R;
-    for (int e = threadIdx.x; e < n * n; e += WG) s_f[e] = fac[e];
-    double* s_st = s_f + n * n;
-    __syncthreads();
-    const uint64_t row = (uint64_t)blockIdx.x * WG + threadIdx.x;
-    const bool valid = row < rows;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const MT* mrow = M + (valid ? row : 0) * (uint64_t)n;
-    double m[RR ? RR : 1];
-    if constexpr (RR > 0) {
-        #pragma unroll
-        for (int k = 0; k < RR; ++k) m[k] = valid ? (double)mrow[k] : 0.0;
-    }
-    #pragma unroll 1
-    for (int a = 0; a < n; ++a) {
-        const double* f = s_f + a * n;
+    constexpr int SLOTS = WG / G;
+    __shared__ double s_m[SLOTS][G + 1];
+    __shared__ double s_st[SLOTS][G];
+    const int j = threadIdx.x % G, slot = threadIdx.x / G;
+    const bool col = j < R;
+    double f[G];
+    #pragma unroll
+    for (int k = 0; k < G; ++k) f[k] = (col && k < R) ? fac[j * R + k] : 0.0;
+    double st = which == 0 ? 0.0 : 1.0;
+    const uint64_t step = (uint64_t)gridDim.x * SLOTS;
+    // every lane runs the same number of iterations: the barriers below are uniform
+    const uint64_t iters = (rows + step - 1) / step;
+    for (uint64_t it = 0; it < iters; ++it) {
+        const uint64_t row = it * step + (uint64_t)blockIdx.x * SLOTS + slot;
+        const bool ok = row < rows && col;
+        s_m[slot][j] = ok ? (double)M[row * (uint64_t)R + j] : 0.0;
+        __syncthreads();
         double t = 0.0;
-        if constexpr (RR > 0) {
-            #pragma unroll
-            for (int k = 0; k < RR; ++k) t += f[k] * m[k];
-        } else {
-            for (int k = 0; k < n; ++k) t += f[k] * (valid ? (double)mrow[k] : 0.0);
-        }
-        if (valid) U[row * (uint64_t)n + a] = from_f64<UT>(t);
-        double x = (which == 0) ? (valid ? t * t : 0.0) : (valid ? fmax(t, 1.0) : 1.0);
         #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            double y = __shfl_xor(x, o);
-            x = (which == 0) ? x + y : fmax(x, y);
+        for (int k = 0; k < G; ++k) t += f[k] * s_m[slot][k];
+        if (ok) {
+            U[row * (uint64_t)R + j] = from_f64<UT>(t);
+            st = which == 0 ? st + t * t : fmax(st, t);
         }
-        if (lane == 0) s_st[w * n + a] = x;
+        __syncthreads();
     }
+    s_st[slot][j] = st;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += WG) {
-        double x = s_st[i];
-        for (int ww = 1; ww < WAVES_PER_WG; ++ww)
-            x = (which == 0) ? x + s_st[ww * n + i] : fmax(x, s_st[ww * n + i]);
-        stats[(uint64_t)blockIdx.x * n + i] = x;
+    if (threadIdx.x < G && col) {
+        double x = s_st[0][j];
+        for (int q = 1; q < SLOTS; ++q) x = which == 0 ? x + s_st[q][j] : fmax(x, s_st[q][j]);
+        stats[(uint64_t)blockIdx.x * R + j] = x;
     }
 }
 
-// lambda and 1/lambda from per-workgroup statistics.
+// lambda and 1/lambda from per-workgroup statistics: lane t sums the parts
+// p = t / R, t / R + WG / R, ... of column t % R; the partial results are
+// combined in a fixed order (deterministic).
 __global__ __launch_bounds__(WG) void k_lambda(const double* __restrict__ stats, int nparts, int R,
                                                int which, double* __restrict__ lambda,
                                                double* __restrict__ inv)
 {
-    for (int r = threadIdx.x; r < R; r += WG) {
-        double x = (which == 0) ? 0.0 : 1.0;
-        for (int p = 0; p < nparts; ++p) {
-            double y = stats[(uint64_t)p * R + r];
+    __shared__ double s_x[WG];
+    const int per = R <= WG ? WG / R : 1;  // lanes per column
+    const int r = threadIdx.x % R, q = threadIdx.x / R;
+    double x = (which == 0) ? 0.0 : 1.0;
+    if (q < per && r < R)
+        for (int p = q; p < nparts; p += per) {
+            const double y = stats[(uint64_t)p * R + r];
             x = (which == 0) ? x + y : fmax(x, y);
         }
-        if (which == 0) x = sqrt(x);
-        lambda[r] = x;
-        inv[r] = (x == 0.0) ? 0.0 : 1.0 / x;
+    s_x[threadIdx.x] = x;
+    __syncthreads();
+    for (int c = threadIdx.x; c < R; c += WG) {
+        double y = (which == 0) ? 0.0 : 1.0;
+        for (int k = 0; k < per && k * R + c < WG; ++k) {
+            const double z = s_x[k * R + c];
+            y = (which == 0) ? y + z : fmax(y, z);
+        }
+        if (which == 0) y = sqrt(y);
+        lambda[c] = y;
+        inv[c] = (y == 0.0) ? 0.0 : 1.0 / y;
     }
 }
 
@@ -520,7 +529,7 @@ int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg)
     uint64_t steps = (rows + 3) / 4;
     uint64_t want = (steps + 63) / 64;  // ~64 row-steps per workgroup (16 per wave)
     if (want < 1) want = 1;
-    if (want > 512) want = 512;  // callers size partial buffers for 512 parts
+    if (want > GRAM_MAX_PARTS) want = GRAM_MAX_PARTS;  // callers size partial buffers for this
     uint64_t rpw = (rows + want - 1) / want;
     rpw = (rpw + 3) / 4 * 4;
     if (rpw == 0) rpw = 4;
@@ -563,24 +572,17 @@ template <typename MT, typename UT>
 static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac,
                         int which, double* stats, hipStream_t st)
 {
-    uint32_t grid = (uint32_t)((rows + WG - 1) / WG);
-    if (grid == 0) return BLCO_OK;
-    size_t lds = sizeof(double) * ((size_t)R * R + (size_t)WAVES_PER_WG * R);
-    const void* k;
-    switch (R) {
-        case 8: k = (const void*)&k_solve_rows<MT, UT, 8>; break;
-        case 16: k = (const void*)&k_solve_rows<MT, UT, 16>; break;
-        case 32: k = (const void*)&k_solve_rows<MT, UT, 32>; break;
-        case 64: k = (const void*)&k_solve_rows<MT, UT, 64>; break;
-        default:
-            if (R > 128) return fail(BLCO_EUNSUPPORTED, "rank %d > 128 in the row solve", R);
-            k = (const void*)&k_solve_rows<MT, UT, 0>;
-            break;
+    if (rows == 0) return BLCO_OK;
+    const uint32_t grid = solve_parts(rows);
+    if (R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d > 64 in the row solve", R);
+    const int g = R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64;
+    switch (g) {
+        case 8: hipLaunchKernelGGL((k_solve_rows<MT, UT, 8>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        case 16: hipLaunchKernelGGL((k_solve_rows<MT, UT, 16>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        case 32: hipLaunchKernelGGL((k_solve_rows<MT, UT, 32>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        default: hipLaunchKernelGGL((k_solve_rows<MT, UT, 64>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
     }
-    if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    void* params[] = {(void*)&M, (void*)&U, (void*)&rows, (void*)&R, (void*)&fac,
-                      (void*)&which, (void*)&stats};
-    HIP_TRY(hipLaunchKernel(k, dim3(grid), dim3(WG), params, lds, st));
+    HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }
 
@@ -592,7 +594,12 @@ int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, c
     return solve_rows_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, which, stats, st);
 }
 
-uint32_t solve_parts(uint64_t rows) { return (uint32_t)((rows + WG - 1) / WG); }
+// Workgroups of the row solve (each walks rows with a grid stride).
+uint32_t solve_parts(uint64_t rows)
+{
+    const uint64_t p = (rows + 31) / 32;
+    return (uint32_t)(p < 1 ? 1 : p > 2048 ? 2048 : p);
+}
 
 int launch_lambda(const double* stats, int nparts, int R, int which, double* lambda, double* inv,
                   hipStream_t st)

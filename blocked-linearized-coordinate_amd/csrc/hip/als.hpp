@@ -13,6 +13,7 @@ struct GramSet {
     const double* g[BLCO_MAX_MODES];
 };
 
+constexpr int GRAM_MAX_PARTS = 2048;  // partial Gram blocks (workgroups) at most
 int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg);
 // G = U^T U (f64, R x R) from dtype U; with inv_scale, U is first scaled in
 // place by inv_scale[col]; with M, ip[col] = sum_i U_ic M_ic.  part needs

@@ -191,7 +191,7 @@ int blco_cpals_create(blco_dev* d, const blco_cpals_opts* o, const double* const
         if (hipEventCreate(&s->ev0[m]) != hipSuccess || hipEventCreate(&s->ev1[m]) != hipSuccess)
             return bad(fail(BLCO_EHIP, "event creation failed"));
     }
-    const int gparts = 512;  // upper bound of gram_parts() for any row count
+    const int gparts = GRAM_MAX_PARTS;  // upper bound of gram_parts() for any row count
     uint32_t sparts = solve_parts(maxI);
     const int nnorm = 1024;
     if (hipMalloc(&s->Mbuf, maxI * R * msz) != hipSuccess ||

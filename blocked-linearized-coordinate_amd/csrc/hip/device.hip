@@ -379,9 +379,11 @@ static int ensure_row_ordered(blco_dev* d, int mode)
     const size_t vsz = value_size(d->vdtype);
     std::vector<uint64_t> coords(n), perm(n), sc(n);
     std::vector<char> vals(n * vsz), sv(n * vsz);
+    progress("row-ordered copy of mode %d: %llu nonzeros", mode, (unsigned long long)n);
     HIP_TRY(hipMemcpy(coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(vals.data(), d->d_vals, n * vsz, hipMemcpyDeviceToHost));
     row_order(mode, d->masks, d->pos, n, coords.data(), d->nblocks, d->bstart.data(), d->bnnz.data(), perm.data());
+    progress("row-ordered copy of mode %d: sorted", mode);
     gather_bytes(coords.data(), 8, perm.data(), n, sc.data());
     gather_bytes(vals.data(), vsz, perm.data(), n, sv.data());
     uint64_t* dc = nullptr;
@@ -831,6 +833,8 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
             fblk.push_back((uint32_t)b);
         }
     const uint64_t nfine = fbeg.size();
+    progress("device tensor: checking %llu nonzeros in %llu blocks", (unsigned long long)nnz,
+             (unsigned long long)nblocks);
     // pass 1: bounds check and per-mode row counts
     std::vector<std::vector<uint32_t>> hot(nmodes);
     std::vector<std::vector<uint64_t>> hot_cnt(nmodes);
@@ -907,6 +911,7 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
     d->nblocks = nblocks;
     d->bstart.assign(block_start, block_start + nblocks);
     d->bnnz.assign(block_nnz, block_nnz + nblocks);
+    progress("device tensor: selector features");
     compute_static_features(nmodes, d->dims, d->masks, d->pos, nnz, coords, nblocks, block_start, block_nnz,
                             block_coords, BLCO_SEL_NFEAT, &d->sfeat[0][0]);
     d->fine_begin.swap(fbeg);

@@ -29,6 +29,19 @@ typedef unsigned __int128 u128;
 
 namespace blco {
 
+void progress(const char* fmt, ...)
+{
+    static const bool on = getenv("BLCO_PROGRESS") != nullptr;
+    if (!on) return;
+    va_list ap;
+    va_start(ap, fmt);
+    fputs("[blco] ", stderr);
+    vfprintf(stderr, fmt, ap);
+    fputc('\n', stderr);
+    fflush(stderr);
+    va_end(ap);
+}
+
 static thread_local std::string g_err;
 
 int fail(int code, const char* fmt, ...)
@@ -120,7 +133,9 @@ static int encode_impl(const Layout& L, uint64_t nnz, const uint64_t* cidx,
         perm[i] = (P)i;
     }
     if (bad) return fail(BLCO_EINVAL, "a coordinate is outside its mode length");
+    progress("encode: %llu ALTO keys formed, sorting", (unsigned long long)nnz);
     radix_sort(nnz, keys.data(), perm.data(), L.total_bits);
+    progress("encode: keys sorted, re-encoding to BLCO");
     #pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < nnz; ++i) {
         uint64_t klo = (uint64_t)keys[i];

@@ -119,6 +119,7 @@ extern "C" int blco_generate(const blco_gen_spec* sp, blco_coo** out)
             key[d] = k;
             id[d] = d;
         }
+        progress("generate: round %d, %llu draws, sorting", round, (unsigned long long)ndraw);
         radix_sort(ndraw, key.data(), id.data(), total_bits);
         keep.assign(ndraw, 0);
         unique = 0;
@@ -127,6 +128,7 @@ extern "C" int blco_generate(const blco_gen_spec* sp, blco_coo** out)
         if (!sp->topup || unique >= want) break;
     }
     uint64_t nnz = sp->topup ? std::min(unique, want) : unique;
+    progress("generate: %llu unique coordinates, drawing values", (unsigned long long)nnz);
 
     blco_coo* t = new (std::nothrow) blco_coo();
     if (!t) return fail(BLCO_ENOMEM, "out of host memory");

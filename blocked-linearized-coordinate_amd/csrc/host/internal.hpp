@@ -10,6 +10,10 @@
 
 namespace blco {
 
+// A line on stderr when BLCO_PROGRESS is set (long host phases of large
+// tensors: generation, encoding, row-ordered copies).
+void progress(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
 // Sets the thread's last-error message and returns `code`.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
