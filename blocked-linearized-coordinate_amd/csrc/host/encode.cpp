@@ -33,9 +33,10 @@ void progress(const char* fmt, ...)
 {
     static const bool on = getenv("BLCO_PROGRESS") != nullptr;
     if (!on) return;
+    static const double t0 = omp_get_wtime();
     va_list ap;
     va_start(ap, fmt);
-    fputs("[blco] ", stderr);
+    fprintf(stderr, "[blco %8.2f s] ", omp_get_wtime() - t0);
     vfprintf(stderr, fmt, ap);
     fputc('\n', stderr);
     fflush(stderr);
