@@ -61,8 +61,8 @@ static void Usage(const char* call)
             "\t-s or --sparsity     Sparsity of generated tensor\n"
             "\t-c or --check        Check GPU MTTKRP against a sequential COO MTTKRP\n"
             "\t-p or --bench        Run the GPU MTTKRP benchmark\n"
-            "\t-k or --kernel-id    GPU kernel: 1 lvl1, 2 lvl2, 3 lvl3, 10 auto, 11/12 batched,\n"
-            "\t                     13 atomic, 20 per-mode selector\n"
+            "\t-k or --kernel-id    GPU kernel: 1 lvl1, 2 lvl2 (row window), 3 lvl3, 10 auto,\n"
+            "\t                     11/12 batched, 13 atomic, 14 row-ordered, 20 selector\n"
             "\t-n or --num-partitions Number of lvl3 partial matrices\n"
             "\t--device             GPU ordinal\n"
             "\t--thread-cf          Thread coarsening factor (tiles of 64 per wave)\n"
@@ -180,7 +180,7 @@ int main(int argc, char** argv)
             default: Usage(argv[0]); return -1;
         }
     }
-    const int known[] = {1, 2, 3, 10, 11, 12, 13, 20};
+    const int known[] = {1, 2, 3, 10, 11, 12, 13, 14, 20};
     if (kernel_id && std::find(std::begin(known), std::end(known), kernel_id) == std::end(known)) {
         fprintf(stderr, "!ERROR! Unknown kernel %d\n", kernel_id);
         return -1;
