@@ -367,23 +367,100 @@ static int ensure_partials(blco_dev* d, size_t bytes)
     return BLCO_OK;
 }
 
-// One kernel launch over chunks [c0, c1) whose data lives at coords/vals
-// indexed from coord_base.
+// Tiles of the split mode's index for a row-ordered copy: NUM_XCD tiles cut
+// at nonzero quantiles (tile_of[i] for every index i), or 1 when the copy is
+// left untiled:
+//  - small tensors (fewer than XCD_TILE_MIN_NNZ nonzeros);
+//  - a split factor that one XCD's L2 holds whole anyway (below
+//    XCD_TILE_MIN_FACTOR bytes at the rank and factor type of the first call);
+//  - output rows with few nonzeros: tiling cuts every row into up to NUM_XCD
+//    pieces, each flushed to HBM on its own, so it is declined when the added
+//    pieces exceed one per XCD_TILE_RUN_RATIO nonzeros;
+//  - split modes longer than 2^32, or a split index so hot that the largest
+//    tile would hold more than XCD_TILE_SLACK times its share.
+// BLCO_XCD_TILES=0 turns tiling off and =1 forces it (tests).
+constexpr uint64_t XCD_TILE_MIN_NNZ = 1ull << 22;
+constexpr uint64_t XCD_TILE_MIN_FACTOR = 2ull << 20;
+constexpr uint64_t XCD_TILE_RUN_RATIO = 256;
+constexpr double XCD_TILE_SLACK = 1.15;
+
+static int xcd_tiles(const blco_dev* d, int mode, int split, uint64_t row_bytes,
+                     const std::vector<uint64_t>& coords, const std::vector<uint64_t>& bc,
+                     std::vector<uint8_t>* tile_of)
+{
+    const char* env = getenv("BLCO_XCD_TILES");
+    const bool force = env && env[0] == '1', off = env && env[0] == '0';
+    if (off || split < 0) return 1;
+    const uint64_t n = d->nnz, I = d->dims[split], rows = d->dims[mode];
+    if (I > (1ull << 32) || rows > (1ull << 32)) return 1;
+    if (!force && (n < XCD_TILE_MIN_NNZ || I * row_bytes < XCD_TILE_MIN_FACTOR)) return 1;
+    const int N = d->nmodes;
+    auto index = [&](int m, uint64_t b, uint64_t i) {
+        return ((coords[i] >> d->pos[m]) & d->masks[m]) | bc[b * N + m];
+    };
+    std::vector<uint64_t> cnt(I, 0);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (uint64_t b = 0; b < d->nblocks; ++b)
+        for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
+            __atomic_fetch_add(&cnt[index(split, b, i)], 1ull, __ATOMIC_RELAXED);
+    tile_of->assign(I, 0);
+    uint64_t cum = 0, tile_nnz[NUM_XCD] = {};
+    for (uint64_t i = 0; i < I; ++i) {
+        const uint64_t t = std::min<uint64_t>(NUM_XCD - 1, (cum + cnt[i] / 2) * NUM_XCD / n);
+        (*tile_of)[i] = (uint8_t)t;
+        tile_nnz[t] += cnt[i];
+        cum += cnt[i];
+    }
+    if (force) return NUM_XCD;
+    const uint64_t most = *std::max_element(tile_nnz, tile_nnz + NUM_XCD);
+    if ((double)most > XCD_TILE_SLACK * (double)n / NUM_XCD) return 1;
+    // the tiles each output row meets
+    std::vector<uint8_t> met(rows, 0);
+    const uint8_t* to = tile_of->data();
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (uint64_t b = 0; b < d->nblocks; ++b)
+        for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
+            __atomic_fetch_or(&met[index(mode, b, i)], (uint8_t)(1u << to[index(split, b, i)]), __ATOMIC_RELAXED);
+    uint64_t added = 0;
+    #pragma omp parallel for schedule(static) reduction(+ : added)
+    for (uint64_t r = 0; r < rows; ++r)
+        if (met[r]) added += (uint64_t)__builtin_popcount(met[r]) - 1;
+    return added * XCD_TILE_RUN_RATIO <= n ? NUM_XCD : 1;
+}
+
 // Builds the row-ordered copy of a mode (kernel 14) from the resident
-// tensor: download, stable sort of every block by the mode index on the
-// host, upload.  Once per mode.
-static int ensure_row_ordered(blco_dev* d, int mode)
+// tensor: download, stable sort on the host, upload.  Once per mode.  The
+// copy is grouped by (tile of the split mode, block) and sorted by the mode
+// index within a group, where the split mode is the longest other mode: the
+// chunks of tile t all run on XCD t, whose L2 then holds only 1/NUM_XCD of
+// the split mode's factor rows.
+static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
 {
     if (d->d_scoords[mode] || !d->nnz) return BLCO_OK;
-    const uint64_t n = d->nnz;
+    const uint64_t n = d->nnz, nb = d->nblocks;
+    const int N = d->nmodes;
     const size_t vsz = value_size(d->vdtype);
-    std::vector<uint64_t> coords(n), perm(n), sc(n);
+    std::vector<uint64_t> coords(n), perm(n), sc(n), bc(nb * N);
     std::vector<char> vals(n * vsz), sv(n * vsz);
     progress("row-ordered copy of mode %d: %llu nonzeros", mode, (unsigned long long)n);
     HIP_TRY(hipMemcpy(coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(vals.data(), d->d_vals, n * vsz, hipMemcpyDeviceToHost));
-    row_order(mode, d->masks, d->pos, n, coords.data(), d->nblocks, d->bstart.data(), d->bnnz.data(), perm.data());
-    progress("row-ordered copy of mode %d: sorted", mode);
+    HIP_TRY(hipMemcpy(bc.data(), d->d_bcoords, nb * N * 8, hipMemcpyDeviceToHost));
+    int split = -1;
+    for (int m = 0; m < N; ++m)
+        if (m != mode && (split < 0 || d->dims[m] > d->dims[split])) split = m;
+    std::vector<uint8_t> tile_of;
+    const int ntiles = xcd_tiles(d, mode, split, (uint64_t)rank * dtype_size(ftype), coords, bc, &tile_of);
+    std::vector<uint64_t> gnnz;
+    if (ntiles > 1) {
+        gnnz.resize((uint64_t)ntiles * nb);
+        row_order_tiled(mode, split, tile_of.data(), ntiles, N, d->masks, d->pos, bc.data(), n, coords.data(),
+                        nb, d->bstart.data(), d->bnnz.data(), perm.data(), gnnz.data());
+    } else {
+        gnnz = d->bnnz;
+        row_order(mode, d->masks, d->pos, n, coords.data(), nb, d->bstart.data(), d->bnnz.data(), perm.data());
+    }
+    progress("row-ordered copy of mode %d: sorted (%d tiles of mode %d)", mode, ntiles, split);
     gather_bytes(coords.data(), 8, perm.data(), n, sc.data());
     gather_bytes(vals.data(), vsz, perm.data(), n, sv.data());
     uint64_t* dc = nullptr;
@@ -402,38 +479,49 @@ static int ensure_row_ordered(blco_dev* d, int mode)
     d->d_scoords[mode] = dc;
     d->d_svals[mode] = dv;
     d->bytes += n * (8 + vsz);
-    // run starts of equal (block, row), for the row-aware chunk tables
+    d->stiles[mode] = ntiles;
+    // groups in storage order, and run starts of equal (group, row)
+    std::vector<SGroup>& groups = d->sgrp_h[mode];
     std::vector<uint64_t>& runs = d->srun_h[mode];
+    groups.clear();
     runs.clear();
     const uint64_t mk = d->masks[mode];
     const int ps = d->pos[mode];
-    for (uint64_t b = 0; b < d->nblocks; ++b)
-        for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
-            if (i == d->bstart[b] || ((sc[i] >> ps) & mk) != ((sc[i - 1] >> ps) & mk)) runs.push_back(i);
+    uint64_t start = 0;
+    for (uint64_t g = 0; g < gnnz.size(); ++g) {
+        if (!gnnz[g]) continue;
+        groups.push_back(SGroup{start, gnnz[g], (uint32_t)(g % nb), (uint32_t)(g / nb)});
+        for (uint64_t i = start; i < start + gnnz[g]; ++i)
+            if (i == start || ((sc[i] >> ps) & mk) != ((sc[i - 1] >> ps) & mk)) runs.push_back(i);
+        start += gnnz[g];
+    }
     runs.push_back(n);
     return BLCO_OK;
 }
 
-// Chunks of the row-ordered copy of a mode: within a block, a chunk closes at
+// Chunks of the row-ordered copy of a mode: within a group, a chunk closes at
 // the first row boundary once it holds 64 * tpw nonzeros, and never holds more
 // than twice that (a long row is cut), so most rows end inside a chunk while
-// every chunk stays short enough to spread a hot row over many waves.
+// every chunk stays short enough to spread a hot row over many waves.  A tiled
+// copy pads every tile's chunk list with empty chunks to one common length, a
+// multiple of the waves per workgroup: the grid then has NUM_XCD equal ranges
+// of workgroups and the XCD-contiguous workgroup numbering puts tile t on XCD t.
 static const ChunkTab* sorted_chunk_table(blco_dev* d, int mode, int tpw)
 {
     for (auto& t : d->stabs[mode])
         if (t.tpw == tpw) return &t;
     const uint64_t base = (uint64_t)WAVE * tpw, cap = 2 * base;
-    std::vector<Chunk> h;
+    const int ntiles = std::max(1, d->stiles[mode]);
+    std::vector<std::vector<Chunk>> tiles(ntiles);
     const std::vector<uint64_t>& runs = d->srun_h[mode];
     size_t r = 0;
-    for (uint64_t b = 0; b < d->nblocks; ++b) {
-        const uint64_t end = d->bstart[b] + d->bnnz[b];
-        uint64_t c0 = d->bstart[b];
-        while (r < runs.size() && runs[r] < d->bstart[b]) ++r;
-        auto emit = [&](uint64_t lo, uint64_t hi) {
-            h.push_back(Chunk{lo, (uint32_t)(hi - lo), (uint32_t)b});
-        };
-        // walk the runs of this block: [runs[r], runs[r+1]) clipped to the block
+    for (const SGroup& g : d->sgrp_h[mode]) {
+        std::vector<Chunk>& h = tiles[g.tile];
+        const uint64_t end = g.start + g.nnz;
+        uint64_t c0 = g.start;
+        while (r < runs.size() && runs[r] < g.start) ++r;
+        auto emit = [&](uint64_t lo, uint64_t hi) { h.push_back(Chunk{lo, (uint32_t)(hi - lo), g.block}); };
+        // walk the runs of this group: [runs[r], runs[r+1]) clipped to the group
         while (c0 < end) {
             const uint64_t run_end = (r + 1 < runs.size()) ? std::min(runs[r + 1], end) : end;
             if (run_end - c0 > cap) {  // inside a long row
@@ -450,6 +538,19 @@ static const ChunkTab* sorted_chunk_table(blco_dev* d, int mode, int tpw)
                 if (c0 < end) emit(c0, end);
                 c0 = end;
             }
+        }
+    }
+    std::vector<Chunk> h;
+    if (ntiles == 1) {
+        h.swap(tiles[0]);
+    } else {
+        size_t most = 0;
+        for (auto& tl : tiles) most = std::max(most, tl.size());
+        most = (most + WAVES_PER_WG - 1) / WAVES_PER_WG * WAVES_PER_WG;
+        h.reserve(most * ntiles);
+        for (auto& tl : tiles) {
+            h.insert(h.end(), tl.begin(), tl.end());
+            h.resize(h.size() + (most - tl.size()), Chunk{0, 0, 0});
         }
     }
     ChunkTab t;
@@ -471,6 +572,8 @@ static const ChunkTab* sorted_chunk_table(blco_dev* d, int mode, int tpw)
     return &d->stabs[mode].back();
 }
 
+// One kernel launch over chunks [c0, c1) whose data lives at coords/vals
+// indexed from coord_base.
 static int launch_range(blco_dev* d, int which, MttkrpArgs a, const ChunkTab* t, uint32_t c0,
                         uint32_t c1, int ftype, hipStream_t st)
 {
@@ -550,7 +653,7 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     if (used_kernel) *used_kernel = kernel;
     if (d->nnz == 0) return BLCO_OK;
     if (which == W_SORTED) {
-        int rc = ensure_row_ordered(d, mode);
+        int rc = ensure_row_ordered(d, mode, rank, ftype);
         if (rc == BLCO_ENOMEM && chosen) {  // no room for the copy: the row window
             d->sorted_failed[mode] = true;
             which = W_WINDOW;

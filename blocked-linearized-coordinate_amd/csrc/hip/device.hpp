@@ -61,6 +61,13 @@ struct ChunkTab {
     std::vector<StreamUnit> units;
 };
 
+// A contiguous run of a row-ordered copy: the nonzeros of one BLCO block
+// whose split-mode index falls in one tile.
+struct SGroup {
+    uint64_t start, nnz;
+    uint32_t block, tile;
+};
+
 }  // namespace blco
 
 struct blco_dev {
@@ -96,7 +103,11 @@ struct blco_dev {
     uint64_t* d_scoords[BLCO_MAX_MODES] = {};
     void* d_svals[BLCO_MAX_MODES] = {};
     bool sorted_failed[BLCO_MAX_MODES] = {};  // no device memory for the copy
-    // positions where a run of equal (block, row) starts in the copy, then nnz
+    // groups of the copy in storage order (tile-major), the number of tiles
+    // (1: untiled) and the positions where a run of equal (group, row)
+    // starts, then nnz
+    std::vector<blco::SGroup> sgrp_h[BLCO_MAX_MODES];
+    int stiles[BLCO_MAX_MODES] = {};
     std::vector<uint64_t> srun_h[BLCO_MAX_MODES];
     std::list<blco::ChunkTab> stabs[BLCO_MAX_MODES];  // row-aware chunks of the copy, per tpw
     void* part = nullptr;  // partial rows of the row-window MTTKRP

@@ -44,6 +44,13 @@ int make_layout(int nmodes, const uint64_t* dims, Layout* L);
 void row_order(int mode, const uint64_t* masks, const int* pos, uint64_t nnz, const uint64_t* coords,
                uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm);
 void gather_bytes(const void* src, size_t esz, const uint64_t* perm, uint64_t n, void* dst);
+// Stable permutation ordering the nonzeros by (tile of the split mode's index,
+// block, mode index); group_nnz[tile * nblocks + block] receives the group
+// sizes.  ntiles <= 1: no split (groups = blocks).
+void row_order_tiled(int mode, int split, const uint8_t* tile_of, int ntiles, int N, const uint64_t* masks,
+                     const int* pos, const uint64_t* bcoords, uint64_t nnz, const uint64_t* coords,
+                     uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm,
+                     uint64_t* group_nnz);
 
 // Kernel-selector features of every mode that depend on the tensor only
 // (entries 0-3, 5, 7-22 of each row of `stride` floats; features.cpp).

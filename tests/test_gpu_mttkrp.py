@@ -96,6 +96,16 @@ def test_row_ordered_all_shapes(B, O, gpu, tensors, name):
     check(B, O, tensors, name, 0, B.K_SORTED, 3, tpw=1)
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_row_ordered_xcd_tiles(B, O, gpu, tensors, name, monkeypatch):
+    """Kernel 14 over copies grouped by 8 tiles of the split mode (forced on
+    these small tensors, whose tiles are unequal, so the padded chunk lists
+    carry empty chunks)."""
+    monkeypatch.setenv("BLCO_XCD_TILES", "1")
+    check(B, O, tensors, name, 0, B.K_SORTED, 32)
+    check(B, O, tensors, name, 1, B.K_SORTED, 20, tpw=1)
+
+
 @pytest.mark.parametrize("nprtn", [1, 4, 17])
 def test_lvl3_short_modes(B, O, gpu, nprtn):
     dims = (60, 90, 5000)
