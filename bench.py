@@ -132,7 +132,8 @@ def run_large(B, args):
     del coo
     log("BLCO encode %.1f s, %d block(s)" % (time.perf_counter() - t0, h.nblocks))
     U0 = B.init_factors(W["seed"], W["dims"], R)
-    forms = [("fp32", B.F32, 1, False), ("fp64", B.F64, 0, False), ("fp32_streamed", B.F32, 1, True)]
+    forms = [("fp32", B.F32, 1, False), ("fp64", B.F64, 0, False), ("bf16", B.BF16, 1, False),
+             ("fp32_streamed", B.F32, 1, True)]
     if args.large_forms:
         forms = [f for f in forms if f[0] in args.large_forms.split(",")]
     res = {}
@@ -146,7 +147,7 @@ def run_large(B, args):
         dev_bytes = dev.bytes()
         dev.free()
         s_v = 8 if vdt == 0 else 4
-        s_f = 8 if prec == B.F64 else 4
+        s_f = {B.F64: 8, B.F32: 4, B.BF16: 2}[prec]
         per_mode_ms = r["mttkrp_ms"] / max(r["iters"], 1)
         ab = [algorithmic_bytes(W["dims"], nnz, R, s_v, s_f, m) for m in range(3)]
         res[name] = dict(s_per_iter=r["iter_ms"] / max(r["iters"], 1) / 1e3,
@@ -169,9 +170,10 @@ def run_large(B, args):
                    "zipf": list(W["zipf"]), "blocks": int(h.nblocks), "parallelism": "single GPU"},
         "forms": res,
     }
-    if "fp32" in res and "fp64" in res:
-        out["speedup_fp32_vs_fp64"] = round(res["fp64"]["s_per_iter"] / res["fp32"]["s_per_iter"], 3)
-        out["fit_diff_fp32_vs_fp64"] = abs(res["fp32"]["fit"] - res["fp64"]["fit"])
+    for low in ("fp32", "bf16"):
+        if low in res and "fp64" in res:
+            out["speedup_%s_vs_fp64" % low] = round(res["fp64"]["s_per_iter"] / res[low]["s_per_iter"], 3)
+            out["fit_diff_%s_vs_fp64" % low] = abs(res[low]["fit"] - res["fp64"]["fit"])
     print(json.dumps(out), flush=True)
 
 
@@ -179,23 +181,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(workload, kernel_name):
-    """HBM bytes per launch of `kernel_name` for `workload` from the committed
-    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration it records), or None when no summary covers it."""
+def profiled_traffic(workload):
+    """HBM bytes per MTTKRP launch for `workload` from the committed rocprofv3
+    summary: FETCH_SIZE + WRITE_SIZE (scaled by the copy-kernel calibration it
+    records), averaged over the MTTKRP launches of the profiled run (every
+    k_mttkrp_* kernel, weighted by its dispatch count), or None when no
+    summary covers it."""
     path = os.path.join(ROOT, "profiles", "mttkrp_row_ordered_%s_fp32.txt" % workload)
     if not os.path.exists(path):
         return None
     in_traffic = False
+    n_tot, b_tot = 0, 0.0
     for line in open(path):
         if line.startswith("HBM traffic"):
             in_traffic = True
-        elif in_traffic and kernel_name in line:
+        elif in_traffic and "k_mttkrp_" in line:
+            f = line.split()
             try:
-                return round(float(line.split()[-1]) * 1e6)  # total MB per dispatch
-            except ValueError:
+                n = int(f[-4])
+                b_tot += n * float(f[-1]) * 1e6  # total MB per dispatch
+                n_tot += n
+            except (ValueError, IndexError):
                 return None
-    return None
+    return round(b_tot / n_tot) if n_tot else None
 
 
 def algorithmic_bytes(dims, nnz, rank, s_v, s_f, mode):
@@ -243,7 +251,7 @@ def main():
                     choices=sorted(WORKLOADS) + ["batched-64", "large-1.7B"])
     ap.add_argument("--large-scale", type=float, default=1.0,
                     help="large-1.7B: fraction of the 1.7e9 nonzeros to generate")
-    ap.add_argument("--large-forms", default="", help="large-1.7B: comma list of fp32,fp64,fp32_streamed")
+    ap.add_argument("--large-forms", default="", help="large-1.7B: comma list of fp32,fp64,bf16,fp32_streamed")
     ap.add_argument("--slot-nnz", type=int, default=1 << 26,
                     help="large-1.7B streamed form: nonzeros per device slot (two slots)")
     ap.add_argument("--cpu-iters", type=int, default=2)
@@ -342,10 +350,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(main_r["mttkrp_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(main_r["mttkrp_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": (profiled_traffic(args.workload, "k_mttkrp_sorted<3, float")
+                     "traffic": (profiled_traffic(args.workload)
                                  if list(main_r["kernels"]) == [14] * len(dims) else None),
                      "traffic_source": "profiles/mttkrp_row_ordered_%s_fp32.txt (FETCH_SIZE x2.000 + "
-                                       "WRITE_SIZE x1.000, calibrated on a 1 GiB copy)" % args.workload,
+                                       "WRITE_SIZE x1.000, calibrated on a 1 GiB copy; mean over the MTTKRP launches)" % args.workload,
                      "kernel": "MTTKRP launches of one FP32 iteration (kernel ids %s per mode); "
                                "algorithmic bytes nnz*(8+4) + sum_m I_m*R*4 per launch"
                                % main_r["kernels"].tolist()},

@@ -383,6 +383,18 @@ constexpr uint64_t XCD_TILE_MIN_NNZ = 1ull << 22;
 constexpr uint64_t XCD_TILE_MIN_FACTOR = 2ull << 20;
 constexpr uint64_t XCD_TILE_RUN_RATIO = 256;
 constexpr double XCD_TILE_SLACK = 1.15;
+// The fiber kernel runs on a fiber-ordered copy whose fibers average at
+// least this many nonzeros (below, the plain row-ordered kernel runs on it).
+// BLCO_FIBERS=0 keeps the ALTO order within rows (no fibers), =1 runs the
+// fiber kernel whatever the fiber length (tests).
+constexpr double FIBER_MIN_LEN = 1.25;
+// Fiber order is chosen only when the inner factor holds at least this many
+// bytes at the first call's rank and type (smaller ones stay in L2, where
+// the gathers the fibers save are cheap).  The FP64 fiber kernel holds 134
+// VGPRs (3 waves per SIMD against 4 for the plain kernel), so for FP64
+// factors it pays only once most inner gathers miss the caches.
+constexpr uint64_t FIBER_MIN_INNER = 2ull << 20;
+constexpr uint64_t FIBER_MIN_INNER_F64 = 64ull << 20;
 
 static int xcd_tiles(const blco_dev* d, int mode, int split, uint64_t row_bytes,
                      const std::vector<uint64_t>& coords, const std::vector<uint64_t>& bc,
@@ -446,45 +458,56 @@ static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
     HIP_TRY(hipMemcpy(coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(vals.data(), d->d_vals, n * vsz, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bc.data(), d->d_bcoords, nb * N * 8, hipMemcpyDeviceToHost));
+    // fiber order: the inner mode is the longest input mode, the others are
+    // outer; the split mode for the XCD tiles is then the longest outer mode
+    // (a split of the inner mode would cut the fibers), else (ALTO order
+    // within rows) the longest input mode.
+    RowOrderSpec o;
+    o.mode = mode;
+    o.nmodes = N;
+    o.masks = d->masks;
+    o.pos = d->pos;
+    o.nblocks = nb;
+    o.block_start = d->bstart.data();
+    o.block_nnz = d->bnnz.data();
+    o.bcoords = bc.data();
+    int inner = -1;
+    const char* fenv = getenv("BLCO_FIBERS");
+    if (N >= 3 && !(fenv && fenv[0] == '0')) {
+        for (int m = 0; m < N; ++m)
+            if (m != mode && (inner < 0 || d->dims[m] > d->dims[inner])) inner = m;
+        int bits = 0;
+        for (int m = 0; m < N; ++m)
+            if (m != mode && m != inner) {
+                o.outer[o.nouter++] = m;
+                bits += d->masks[m] ? 64 - __builtin_clzll(d->masks[m]) : 0;
+            }
+        // fibers save inner-factor gathers, which pay when that factor is
+        // too large for one XCD's L2
+        const bool big = d->dims[inner] * (uint64_t)rank * dtype_size(ftype) >=
+                         (ftype == BLCO_F64 ? FIBER_MIN_INNER_F64 : FIBER_MIN_INNER);
+        if (bits > 64 || !(big || (fenv && fenv[0] == '1'))) {
+            inner = -1;
+            o.nouter = 0;
+        }
+    }
     int split = -1;
     for (int m = 0; m < N; ++m)
-        if (m != mode && (split < 0 || d->dims[m] > d->dims[split])) split = m;
+        if (m != mode && m != inner && (split < 0 || d->dims[m] > d->dims[split])) split = m;
     std::vector<uint8_t> tile_of;
     const int ntiles = xcd_tiles(d, mode, split, (uint64_t)rank * dtype_size(ftype), coords, bc, &tile_of);
-    std::vector<uint64_t> gnnz;
-    if (ntiles > 1) {
-        gnnz.resize((uint64_t)ntiles * nb);
-        row_order_tiled(mode, split, tile_of.data(), ntiles, N, d->masks, d->pos, bc.data(), n, coords.data(),
-                        nb, d->bstart.data(), d->bnnz.data(), perm.data(), gnnz.data());
-    } else {
-        gnnz = d->bnnz;
-        row_order(mode, d->masks, d->pos, n, coords.data(), nb, d->bstart.data(), d->bnnz.data(), perm.data());
-    }
-    progress("row-ordered copy of mode %d: sorted (%d tiles of mode %d)", mode, ntiles, split);
+    o.split = split;
+    o.ntiles = ntiles;
+    o.tile_of = tile_of.data();
+    std::vector<uint64_t> gnnz((uint64_t)ntiles * nb);
+    row_order(o, n, coords.data(), perm.data(), gnnz.data());
+    progress("row-ordered copy of mode %d: sorted (%d tiles of mode %d, inner mode %d)", mode, ntiles, split,
+             inner);
     gather_bytes(coords.data(), 8, perm.data(), n, sc.data());
     gather_bytes(vals.data(), vsz, perm.data(), n, sv.data());
-    uint64_t* dc = nullptr;
-    void* dv = nullptr;
-    if (hipMalloc(&dc, n * 8) != hipSuccess || hipMalloc(&dv, n * vsz) != hipSuccess) {
-        if (dc) (void)hipFree(dc);
-        return fail(BLCO_ENOMEM, "no device memory for the row-ordered copy of mode %d (%llu bytes)", mode,
-                    (unsigned long long)(n * (8 + vsz)));
-    }
-    if (hipMemcpy(dc, sc.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dv, sv.data(), n * vsz, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(dc);
-        (void)hipFree(dv);
-        return fail(BLCO_EHIP, "upload of the row-ordered copy failed");
-    }
-    d->d_scoords[mode] = dc;
-    d->d_svals[mode] = dv;
-    d->bytes += n * (8 + vsz);
-    d->stiles[mode] = ntiles;
     // groups in storage order, and run starts of equal (group, row)
-    std::vector<SGroup>& groups = d->sgrp_h[mode];
-    std::vector<uint64_t>& runs = d->srun_h[mode];
-    groups.clear();
-    runs.clear();
+    std::vector<SGroup> groups;
+    std::vector<uint64_t> runs;
     const uint64_t mk = d->masks[mode];
     const int ps = d->pos[mode];
     uint64_t start = 0;
@@ -496,6 +519,59 @@ static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
         start += gnnz[g];
     }
     runs.push_back(n);
+    // fibers: positions where the group, the row or an outer index changes
+    uint64_t fibers = 0;
+    if (inner >= 0) {
+        uint64_t fm[BLCO_MAX_MODES];
+        int fp[BLCO_MAX_MODES], nf = 0;
+        fm[nf] = mk;
+        fp[nf++] = ps;
+        for (int j = 0; j < o.nouter; ++j) {
+            fm[nf] = d->masks[o.outer[j]];
+            fp[nf++] = d->pos[o.outer[j]];
+        }
+        const int64_t ng = (int64_t)groups.size();
+        #pragma omp parallel for schedule(dynamic, 1) reduction(+ : fibers)
+        for (int64_t gi = 0; gi < ng; ++gi) {
+            const uint64_t lo = groups[gi].start, hi = lo + groups[gi].nnz;
+            uint64_t f = 1;
+            for (uint64_t i = lo + 1; i < hi; ++i) {
+                bool first = false;
+                for (int k = 0; k < nf && !first; ++k)
+                    first = ((sc[i] >> fp[k]) & fm[k]) != ((sc[i - 1] >> fp[k]) & fm[k]);
+                f += first;
+            }
+            fibers += f;
+        }
+    }
+    const bool use_fibers = inner >= 0 && ((fenv && fenv[0] == '1') || (double)n >= FIBER_MIN_LEN * (double)fibers);
+    if (inner >= 0)
+        progress("row-ordered copy of mode %d: %.2f nonzeros per fiber%s", mode, (double)n / (double)fibers,
+                 use_fibers ? "" : " (fiber kernel off)");
+    uint64_t* dc = nullptr;
+    void* dv = nullptr;
+    auto release = [&]() {
+        if (dc) (void)hipFree(dc);
+        if (dv) (void)hipFree(dv);
+    };
+    if (hipMalloc(&dc, n * 8) != hipSuccess || hipMalloc(&dv, n * vsz) != hipSuccess) {
+        release();
+        return fail(BLCO_ENOMEM, "no device memory for the row-ordered copy of mode %d (%llu bytes)", mode,
+                    (unsigned long long)(n * (8 + vsz)));
+    }
+    if (hipMemcpy(dc, sc.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dv, sv.data(), n * vsz, hipMemcpyHostToDevice) != hipSuccess) {
+        release();
+        return fail(BLCO_EHIP, "upload of the row-ordered copy failed");
+    }
+    d->d_scoords[mode] = dc;
+    d->d_svals[mode] = dv;
+    d->bytes += n * (8 + vsz);
+    d->stiles[mode] = ntiles;
+    d->sinner[mode] = inner;
+    d->sfiber[mode] = use_fibers;
+    d->sgrp_h[mode].swap(groups);
+    d->srun_h[mode].swap(runs);
     return BLCO_OK;
 }
 
@@ -784,7 +860,12 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         // 32-bit row offsets when every factor's rank-wide element offset fits
         bool wide = false;
         for (int m = 0; m < d->nmodes; ++m) wide |= d->dims[m] * (uint64_t)rank > 0xFFFFFFFFull;
-        rc = launch_range(d, wide ? W_SORTED64 : W_SORTED, a, stab, 0, stab->n, ftype, st);
+        int kind = wide ? W_SORTED64 : W_SORTED;
+        if (d->sfiber[mode]) {
+            kind = wide ? W_FIBER64 : W_FIBER;
+            a.inner = d->sinner[mode];
+        }
+        rc = launch_range(d, kind, a, stab, 0, stab->n, ftype, st);
     } else if (!d->stream_data) {
         a.coords = d->d_coords;
         a.vals = d->d_vals;

@@ -108,6 +108,10 @@ struct blco_dev {
     // starts, then nnz
     std::vector<blco::SGroup> sgrp_h[BLCO_MAX_MODES];
     int stiles[BLCO_MAX_MODES] = {};
+    // fiber order of the copy: within a row by every input mode but sinner;
+    // sfiber: the fiber kernel runs on it (fibers long enough)
+    int sinner[BLCO_MAX_MODES] = {};
+    bool sfiber[BLCO_MAX_MODES] = {};
     std::vector<uint64_t> srun_h[BLCO_MAX_MODES];
     std::list<blco::ChunkTab> stabs[BLCO_MAX_MODES];  // row-aware chunks of the copy, per tpw
     void* part = nullptr;  // partial rows of the row-window MTTKRP

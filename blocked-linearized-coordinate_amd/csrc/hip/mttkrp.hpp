@@ -54,9 +54,12 @@ struct MttkrpArgs {
     const uint64_t* sc_poff;   // [entry + 1]: the entry's window is partial rows [poff[e], poff[e+1])
     uint64_t hot_poff;         // first partial row of the hot areas (entry e: + e * hot_rows)
     void* part;                // partial rows (rank wide), reduced by k_window_reduce
+    // fiber kernel only: the input mode gathered per nonzero
+    int inner;
 };
 
-enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4, W_SORTED = 5, W_SORTED64 = 6 };
+enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4, W_SORTED = 5, W_SORTED64 = 6,
+       W_FIBER = 7, W_FIBER64 = 8 };
 
 constexpr uint32_t FINE_CHUNK = 1024;        // nonzeros per range-table entry
 constexpr uint32_t WINDOW_LDS_BYTES = 40 * 1024;  // hot area + row window per workgroup

@@ -570,6 +570,197 @@ __global__ __launch_bounds__(WG) void k_mttkrp_sorted(MttkrpArgs a)
     }
 }
 
+// ------------------------------------------------------------- fibers
+
+// Loads C columns of a factor row (zero past the rank when not EXACT).
+template <typename FT, int C, bool EXACT>
+__device__ __forceinline__ void gather_row(const FT* f, int col, int R, typename Acc<FT>::type (&x)[C])
+{
+    using AT = typename Acc<FT>::type;
+    if (EXACT || col + C <= R) {
+        load_row<FT, C>(f, x);
+    } else {
+        #pragma unroll
+        for (int cc = 0; cc < C; ++cc) x[cc] = (col + cc < R) ? (AT)to_acc(f[cc]) : (AT)0;
+    }
+}
+
+// MTTKRP over a fiber-ordered copy: the row-ordered copy whose nonzeros are
+// further ordered, within each output row, by the outer modes (every input
+// mode but the inner one, a.inner), so that a fiber -- the nonzeros sharing
+// the output row and all outer indices -- is contiguous.  Per nonzero only
+// the inner factor row is gathered; the outer factor rows are gathered once
+// per fiber, with its first nonzero, and scale the fiber's sum:
+//   out[i] += sum over fibers (prod_outer U_m[i_m]) * (sum v * U_inner[i_inner]).
+// Slots, rounds, the prefetch and the flush are those of k_mttkrp_sorted; a
+// lane also marks whether its nonzero starts a fiber (its key differs from
+// the previous position's, or it is the first of the slot's run).
+template <int N, typename FT, typename VT, int C, int L, bool EXACT, typename IT>
+__global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    constexpr int S = WAVE / L;
+    // two positions per batch: the outer rows of fiber starts need registers
+    // beside the inner rows, and four positions lower the occupancy to 4 waves
+    constexpr int PB = L < 2 ? L : 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = a.chunk0 + xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
+    if (c >= a.chunk0 + a.nchunks) return;  // whole wave leaves; no block barriers below
+    const Chunk ch = a.chunks[c];
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    const int R = a.rank, mode = a.mode, inner = a.inner;
+    const uint32_t run = (ch.len + S - 1) / S;
+    const uint32_t p0 = min((uint32_t)slot * run, ch.len), p1 = min(p0 + run, ch.len);
+    const uint32_t rounds = (run + L - 1) / L;  // the same for every slot
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    const FT* fin = reinterpret_cast<const FT*>(a.f[inner]);
+    for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+        const int col = cb + sub * C;
+        AT acc[C], facc[C], scale[C];
+        #pragma unroll
+        for (int cc = 0; cc < C; ++cc) acc[cc] = facc[cc] = scale[cc] = 0;
+        uint64_t cur = 0;
+        bool have = false;
+        auto flush = [&]() {
+            AT* o = out + cur * (uint64_t)R + col;
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+                if (EXACT || col + cc < R) atomic_add_global(o + cc, acc[cc]);
+        };
+        auto fetch = [&](uint32_t r, uint64_t& coord, AT& v) {
+            const uint32_t q = p0 + r * L + sub;
+            const bool ok = r < rounds && q < p1;
+            const uint64_t el = ch.begin + q - a.coord_base;
+            coord = ok ? stream_load(a.coords + el) : 0;
+            v = ok ? (AT)stream_load(vals + el) : (AT)0;
+        };
+        uint64_t coord;
+        AT v;
+        fetch(0, coord, v);
+        // fiber key of the last position decoded by the slot in the previous round
+        IT carry[N];
+        #pragma unroll
+        for (int m = 0; m < N; ++m) carry[m] = 0;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint32_t base = p0 + r * L;
+            // decode: output row, inner offset, outer offsets (off[m] for
+            // m not the output or inner mode), and the fiber-start mark
+            IT off[N], row = 0, ioff = 0;
+            #pragma unroll
+            for (int m = 0; m < N; ++m) {
+                const uint64_t i = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
+                off[m] = (IT)(i * (uint64_t)R);
+                if (m == mode) row = (IT)i;
+                if (m == inner) ioff = off[m];
+            }
+            bool start = base + sub == p0;
+            #pragma unroll
+            for (int m = 0; m < N; ++m) {
+                if (m == inner) continue;
+                const IT up = __shfl(off[m], sub > 0 ? lane - 1 : lane);
+                start = start || (sub > 0 ? up : carry[m]) != off[m];
+            }
+            #pragma unroll
+            for (int m = 0; m < N; ++m) carry[m] = __shfl(off[m], slot * L + L - 1);
+            const AT vr = v;
+            fetch(r + 1, coord, v);  // in flight during this round's gathers
+            const int nb = base < p1 ? (int)min((uint32_t)L, p1 - base) : 0;
+            #pragma unroll
+            for (int k0 = 0; k0 < L; k0 += PB) {
+                IT rr[PB];
+                bool fs[PB];
+                AT pv[PB][C], po[PB][C];
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    const int src = slot * L + k0 + b;
+                    const bool ok = k0 + b < nb;
+                    rr[b] = __shfl(row, src);
+                    fs[b] = __shfl((int)start, src) != 0;
+                    const AT vq = __shfl(vr, src);
+                    const IT iq = __shfl(ioff, src);
+                    AT x[C];
+                    if (ok) {
+                        gather_row<FT, C, EXACT>(fin + (uint64_t)iq + col, col, R, x);
+                    } else {
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) x[cc] = 0;
+                    }
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) {
+                        pv[b][cc] = vq * x[cc];
+                        po[b][cc] = 1;
+                    }
+                    #pragma unroll
+                    for (int m = 0; m < N; ++m) {
+                        if (m == mode || m == inner) continue;
+                        const IT om = __shfl(off[m], src);
+                        if (ok && fs[b]) {
+                            gather_row<FT, C, EXACT>(reinterpret_cast<const FT*>(a.f[m]) + (uint64_t)om + col, col,
+                                                     R, x);
+                            #pragma unroll
+                            for (int cc = 0; cc < C; ++cc) po[b][cc] *= x[cc];
+                        }
+                    }
+                }
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    if (k0 + b >= nb) break;
+                    if (fs[b]) {
+                        if (have) {
+                            #pragma unroll
+                            for (int cc = 0; cc < C; ++cc) {
+                                acc[cc] += facc[cc] * scale[cc];
+                                facc[cc] = 0;
+                            }
+                            if ((uint64_t)rr[b] != cur) {
+                                flush();
+                                #pragma unroll
+                                for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
+                            }
+                        }
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) scale[cc] = po[b][cc];
+                        cur = rr[b];
+                        have = true;
+                    }
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) facc[cc] += pv[b][cc];
+                }
+            }
+        }
+        #pragma unroll
+        for (int cc = 0; cc < C; ++cc) acc[cc] += facc[cc] * scale[cc];
+        // the slots' last rows: segmented scan over the slots, as in k_mttkrp_sorted
+        const int below = lane - L, above = lane + L;
+        const uint64_t prev_cur = __shfl(cur, below < 0 ? lane : below);
+        const bool prev_have = __shfl(have, below < 0 ? lane : below);
+        bool head = slot == 0 || !have || !prev_have || prev_cur != cur;
+        #pragma unroll
+        for (int d = 1; d < S; d <<= 1) {
+            const int from = lane - d * L < 0 ? lane : lane - d * L;
+            const bool pf = __shfl(head, from);
+            AT pa[C];
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc) pa[cc] = __shfl(acc[cc], from);
+            if (slot >= d) {
+                if (!head) {
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) acc[cc] += pa[cc];
+                }
+                head = head || pf;
+            }
+        }
+        const uint64_t next_cur = __shfl(cur, above >= WAVE ? lane : above);
+        const bool next_have = __shfl(have, above >= WAVE ? lane : above);
+        const bool tail = slot == S - 1 || !next_have || next_cur != cur;
+        if (have && tail) flush();
+    }
+}
+
 // ------------------------------------------------------------- atomic
 
 template <int N, typename FT, typename VT, int C, int L, bool EXACT>
@@ -649,6 +840,12 @@ inline const void* kernel_ptr(int which)
     if (which == W_WINDOW64) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint64_t>;
     if (which == W_SORTED) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint32_t>;
     if (which == W_SORTED64) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint64_t>;
+    if (which == W_FIBER || which == W_FIBER64) {
+        if constexpr (N >= 3)
+            return which == W_FIBER ? (const void*)&k_mttkrp_fiber<N, FT, VT, C, L, EX, uint32_t>
+                                    : (const void*)&k_mttkrp_fiber<N, FT, VT, C, L, EX, uint64_t>;
+        return nullptr;  // an order-2 tensor has no fibers
+    }
     return (const void*)&k_mttkrp_atomic<N, FT, VT, C, L, EX>;
 }
 

@@ -39,18 +39,28 @@ struct Layout {
 // Computes the ALTO / BLCO layout of a shape; returns BLCO_OK or an error.
 int make_layout(int nmodes, const uint64_t* dims, Layout* L);
 
-// Stable permutation ordering every block's nonzeros by their mode index
-// (roworder.cpp), and dst[i] = src[perm[i]] for esz-byte elements.
-void row_order(int mode, const uint64_t* masks, const int* pos, uint64_t nnz, const uint64_t* coords,
-               uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm);
+// The order of a row-ordered copy (roworder.cpp): nonzeros grouped by
+// (tile of the split mode's index, block) -- ntiles <= 1: by block --, by the
+// output mode's index within a group and, with nouter > 0, by the outer modes'
+// local indices within a row (the outer modes' mask bits must total <= 64).
+struct RowOrderSpec {
+    int mode = 0, nmodes = 0;
+    const uint64_t* masks = nullptr;
+    const int* pos = nullptr;
+    uint64_t nblocks = 0;
+    const uint64_t* block_start = nullptr;
+    const uint64_t* block_nnz = nullptr;
+    const uint64_t* bcoords = nullptr;  // nblocks x nmodes
+    int split = -1, ntiles = 1;
+    const uint8_t* tile_of = nullptr;   // tile of every split-mode index
+    int nouter = 0;
+    int outer[BLCO_MAX_MODES] = {};
+};
+// Stable permutation into that order; group_nnz[tile * nblocks + block]
+// receives the group sizes.
+void row_order(const RowOrderSpec& o, uint64_t nnz, const uint64_t* coords, uint64_t* perm, uint64_t* group_nnz);
+// dst[i] = src[perm[i]] for esz-byte elements.
 void gather_bytes(const void* src, size_t esz, const uint64_t* perm, uint64_t n, void* dst);
-// Stable permutation ordering the nonzeros by (tile of the split mode's index,
-// block, mode index); group_nnz[tile * nblocks + block] receives the group
-// sizes.  ntiles <= 1: no split (groups = blocks).
-void row_order_tiled(int mode, int split, const uint8_t* tile_of, int ntiles, int N, const uint64_t* masks,
-                     const int* pos, const uint64_t* bcoords, uint64_t nnz, const uint64_t* coords,
-                     uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm,
-                     uint64_t* group_nnz);
 
 // Kernel-selector features of every mode that depend on the tensor only
 // (entries 0-3, 5, 7-22 of each row of `stride` floats; features.cpp).

@@ -1,12 +1,13 @@
 // roworder.cpp -- the row-ordered copies behind MTTKRP kernel 14.
 //
-// For an output mode n, the nonzeros of every BLCO block are stably sorted by
-// their mode-n index; the BLCO words and values are stored again in that
-// order (blocks keep their extent, so block coordinates and the decode are
-// unchanged).  Equal rows keep the ALTO order, so the other modes' indices
-// stay as local as the BLCO order made them.  The tiled form first groups
-// the nonzeros by a tile of another ("split") mode's index, then by block:
-// group (tile, block) is a contiguous run of the copy.
+// For an output mode n the nonzeros are grouped by (tile of a split mode,
+// BLCO block) -- one tile when the copy is not tiled -- and stably sorted by
+// their mode-n index within a group; the BLCO words and values are stored
+// again in that order.  Every nonzero keeps its block, so block coordinates
+// and the decode are unchanged.  Within a row the nonzeros are either left in
+// ALTO order (which keeps the other modes' indices local) or ordered by the
+// "outer" modes, which makes every fiber (equal row and outer indices)
+// contiguous for the fiber kernel.
 #include <omp.h>
 
 #include <algorithm>
@@ -19,68 +20,65 @@
 
 namespace blco {
 
-void row_order(int mode, const uint64_t* masks, const int* pos, uint64_t nnz, const uint64_t* coords,
-               uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm)
+void row_order(const RowOrderSpec& o, uint64_t nnz, const uint64_t* coords, uint64_t* perm, uint64_t* group_nnz)
 {
-    const uint64_t mk = masks[mode];
-    const int ps = pos[mode];
+    const uint64_t mk = o.masks[o.mode];
+    const int ps = o.pos[o.mode];
     const int row_bits = mk ? 64 - __builtin_clzll(mk) : 0;
-    const int blk_bits = nblocks > 1 ? 64 - __builtin_clzll(nblocks - 1) : 0;
-    if (row_bits + blk_bits <= 64) {
-        // one radix sort on (block, row within the block)
-        std::vector<uint64_t> key(nnz);
-        #pragma omp parallel for schedule(static)
-        for (uint64_t b = 0; b < nblocks; ++b)
-            for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i)
-                key[i] = (b << row_bits) | ((coords[i] >> ps) & mk);
-        #pragma omp parallel for schedule(static)
-        for (uint64_t i = 0; i < nnz; ++i) perm[i] = i;
-        radix_sort(nnz, key.data(), perm, row_bits + blk_bits);
-        return;
-    }
-    // too many key bits: sort each block on its own
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (uint64_t b = 0; b < nblocks; ++b) {
-        uint64_t* p = perm + block_start[b];
-        for (uint64_t i = 0; i < block_nnz[b]; ++i) p[i] = block_start[b] + i;
-        std::stable_sort(p, p + block_nnz[b], [&](uint64_t x, uint64_t y) {
-            return ((coords[x] >> ps) & mk) < ((coords[y] >> ps) & mk);
-        });
-    }
-}
-
-void row_order_tiled(int mode, int split, const uint8_t* tile_of, int ntiles, int N, const uint64_t* masks,
-                     const int* pos, const uint64_t* bcoords, uint64_t nnz, const uint64_t* coords,
-                     uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz, uint64_t* perm,
-                     uint64_t* group_nnz)
-{
-    const uint64_t mk = masks[mode];
-    const int ps = pos[mode];
-    const int row_bits = mk ? 64 - __builtin_clzll(mk) : 0;
-    const uint64_t ngroups = (uint64_t)ntiles * nblocks;
+    const int ntiles = o.ntiles > 1 ? o.ntiles : 1;
+    const uint64_t nb = o.nblocks, ngroups = (uint64_t)ntiles * nb;
     const int grp_bits = ngroups > 1 ? 64 - __builtin_clzll(ngroups - 1) : 0;
-    auto group = [&](uint64_t b, uint64_t i) -> uint64_t {
-        if (ntiles <= 1) return b;
-        const uint64_t si = ((coords[i] >> pos[split]) & masks[split]) | bcoords[b * N + split];
-        return (uint64_t)tile_of[si] * nblocks + b;
-    };
-    std::vector<uint64_t> gid(nnz);
+    // group of every nonzero
+    std::vector<uint64_t> key(nnz);
     #pragma omp parallel for schedule(dynamic, 1)
-    for (uint64_t b = 0; b < nblocks; ++b)
-        for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) gid[i] = group(b, i);
+    for (uint64_t b = 0; b < nb; ++b) {
+        const uint64_t lo = o.block_start[b], hi = lo + o.block_nnz[b];
+        if (ntiles == 1) {
+            for (uint64_t i = lo; i < hi; ++i) key[i] = b;
+        } else {
+            const uint64_t sm = o.masks[o.split], top = o.bcoords[b * o.nmodes + o.split];
+            const int sp = o.pos[o.split];
+            for (uint64_t i = lo; i < hi; ++i)
+                key[i] = (uint64_t)o.tile_of[((coords[i] >> sp) & sm) | top] * nb + b;
+        }
+    }
     for (uint64_t g = 0; g < ngroups; ++g) group_nnz[g] = 0;
-    for (uint64_t i = 0; i < nnz; ++i) ++group_nnz[gid[i]];
+    for (uint64_t i = 0; i < nnz; ++i) ++group_nnz[key[i]];
     #pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < nnz; ++i) perm[i] = i;
-    if (row_bits + grp_bits <= 64) {
-        // one radix sort on (group, row within the block)
+    // within a row: by the outer modes' local indices (LSD: this pass first)
+    if (o.nouter > 0) {
+        std::vector<uint64_t> k2(nnz);
+        int bits = 0;
+        for (int j = 0; j < o.nouter; ++j) {
+            const uint64_t m = o.masks[o.outer[j]];
+            bits += m ? 64 - __builtin_clzll(m) : 0;
+        }
         #pragma omp parallel for schedule(static)
-        for (uint64_t i = 0; i < nnz; ++i) gid[i] = (gid[i] << row_bits) | ((coords[i] >> ps) & mk);
-        radix_sort(nnz, gid.data(), perm, row_bits + grp_bits);
+        for (uint64_t i = 0; i < nnz; ++i) {
+            uint64_t k = 0;
+            for (int j = 0; j < o.nouter; ++j) {
+                const uint64_t m = o.masks[o.outer[j]];
+                const int w = m ? 64 - __builtin_clzll(m) : 0;
+                k = (w < 64 ? k << w : 0) | ((coords[i] >> o.pos[o.outer[j]]) & m);
+            }
+            k2[i] = k;
+        }
+        radix_sort(nnz, k2.data(), perm, bits);
+    }
+    if (row_bits + grp_bits <= 64) {
+        // one stable radix sort on (group, row within the block)
+        std::vector<uint64_t> k1(nnz);
+        #pragma omp parallel for schedule(static)
+        for (uint64_t j = 0; j < nnz; ++j) {
+            const uint64_t i = perm[j];
+            k1[j] = (grp_bits ? key[i] << row_bits : 0) | ((coords[i] >> ps) & mk);
+        }
+        radix_sort(nnz, k1.data(), perm, row_bits + grp_bits);
         return;
     }
     std::stable_sort(perm, perm + nnz, [&](uint64_t x, uint64_t y) {
-        if (gid[x] != gid[y]) return gid[x] < gid[y];
+        if (key[x] != key[y]) return key[x] < key[y];
         return ((coords[x] >> ps) & mk) < ((coords[y] >> ps) & mk);
     });
 }

@@ -106,6 +106,28 @@ def test_row_ordered_xcd_tiles(B, O, gpu, tensors, name, monkeypatch):
     check(B, O, tensors, name, 1, B.K_SORTED, 20, tpw=1)
 
 
+@pytest.mark.parametrize("name", [n for n in CASES if n != "order2"])
+@pytest.mark.parametrize("tiles", ["0", "1"])
+def test_fiber_kernel(B, O, gpu, tensors, name, tiles, monkeypatch):
+    """The fiber kernel (forced on: these small tensors have short fibers)
+    over fiber-ordered copies, untiled and with 8 forced tiles, for every
+    precision and for ranks that do not fill a lane group."""
+    monkeypatch.setenv("BLCO_FIBERS", "1")
+    monkeypatch.setenv("BLCO_XCD_TILES", tiles)
+    check(B, O, tensors, name, 0, B.K_SORTED, 32)
+    check(B, O, tensors, name, 1, B.K_SORTED, 20, tpw=1)
+    check(B, O, tensors, name, 2, B.K_SORTED, 16)
+    check(B, O, tensors, name, 0, B.K_SORTED, 3)
+
+
+def test_fiber_order_without_fiber_kernel(B, O, gpu, tensors, monkeypatch):
+    """The row-ordered kernel over copies kept in ALTO order within rows."""
+    monkeypatch.setenv("BLCO_FIBERS", "0")
+    for name in ("c2shape", "order4", "bits65"):
+        check(B, O, tensors, name, 0, B.K_SORTED, 32)
+        check(B, O, tensors, name, 1, B.K_SORTED, 16)
+
+
 @pytest.mark.parametrize("nprtn", [1, 4, 17])
 def test_lvl3_short_modes(B, O, gpu, nprtn):
     dims = (60, 90, 5000)
