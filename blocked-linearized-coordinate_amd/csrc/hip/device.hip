@@ -372,7 +372,9 @@ static int ensure_partials(blco_dev* d, size_t bytes)
 // left untiled:
 //  - small tensors (fewer than XCD_TILE_MIN_NNZ nonzeros);
 //  - a split factor that one XCD's L2 holds whole anyway (below
-//    XCD_TILE_MIN_FACTOR bytes at the rank and factor type of the first call);
+//    XCD_TILE_MIN_FACTOR bytes at the rank and factor type of the first
+//    call; tiling a smaller split mode, such as a short outer mode of a
+//    fiber-ordered copy, only multiplies the row pieces);
 //  - output rows with few nonzeros: tiling cuts every row into up to NUM_XCD
 //    pieces, each flushed to HBM on its own, so it is declined when the added
 //    pieces exceed one per XCD_TILE_RUN_RATIO nonzeros;
