@@ -37,6 +37,45 @@ __device__ __forceinline__ void load_row(const FT* __restrict__ p,
     }
 }
 
+// A buffer resource over a factor matrix of fewer than 2^32 bytes: a row
+// gather then takes a 32-bit byte offset in one VGPR beside the uniform
+// resource, instead of a 64-bit address.  (Third descriptor word: the raw
+// 32-bit data format of the gfx9 family; unbounded range.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t factor_rsrc(const void* p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+}
+
+template <typename FT, int C>
+__device__ __forceinline__ void load_row_buf(__amdgpu_buffer_rsrc_t rs, uint32_t byte_off,
+                                             typename Acc<FT>::type (&x)[C])
+{
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    FT e[C];
+    if constexpr (C * sizeof(FT) == 32) {
+        const u32x4 u0 = __builtin_amdgcn_raw_buffer_load_b128(rs, byte_off, 0, 0);
+        const u32x4 u1 = __builtin_amdgcn_raw_buffer_load_b128(rs, byte_off + 16, 0, 0);
+        __builtin_memcpy(e, &u0, 16);
+        __builtin_memcpy(reinterpret_cast<char*>(e) + 16, &u1, 16);
+    } else if constexpr (C * sizeof(FT) == 16) {
+        const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(rs, byte_off, 0, 0);
+        __builtin_memcpy(e, &u, 16);
+    } else if constexpr (C * sizeof(FT) == 8) {
+        const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(rs, byte_off, 0, 0);
+        __builtin_memcpy(e, &u, 8);
+    } else if constexpr (C * sizeof(FT) == 4) {
+        const unsigned u = __builtin_amdgcn_raw_buffer_load_b32(rs, byte_off, 0, 0);
+        __builtin_memcpy(e, &u, 4);
+    } else {
+        static_assert(C * sizeof(FT) == 2, "row piece of 2 to 32 bytes");
+        const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(rs, byte_off, 0, 0);
+        __builtin_memcpy(e, &u, 2);
+    }
+    #pragma unroll
+    for (int c = 0; c < C; ++c) x[c] = to_acc(e[c]);
+}
+
 // Tensor coordinates and values are read once per MTTKRP: non-temporal
 // loads keep them from evicting the factor rows that the gathers reuse.
 template <typename T>
@@ -457,6 +496,11 @@ __global__ __launch_bounds__(WG) void k_mttkrp_sorted(MttkrpArgs a)
     const uint32_t rounds = (run + L - 1) / L;  // the same for every slot
     AT* out = reinterpret_cast<AT*>(a.out);
     const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    // 32-bit offsets: every factor holds fewer than 2^32 bytes (host-checked)
+    constexpr bool BUF = sizeof(IT) == 4;
+    __amdgpu_buffer_rsrc_t rs[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) rs[m] = factor_rsrc(a.f[m]);
     for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
         const int col = cb + sub * C;
         AT acc[C];
@@ -511,7 +555,9 @@ __global__ __launch_bounds__(WG) void k_mttkrp_sorted(MttkrpArgs a)
                         AT x[C];
                         if (k0 + b < nb) {
                             const FT* f = reinterpret_cast<const FT*>(a.f[m]) + (uint64_t)om + col;
-                            if (EXACT || col + C <= R) {
+                            if (BUF && (EXACT || col + C <= R)) {
+                                load_row_buf<FT, C>(rs[m], ((uint32_t)om + col) * (uint32_t)sizeof(FT), x);
+                            } else if (EXACT || col + C <= R) {
                                 load_row<FT, C>(f, x);
                             } else {
                                 #pragma unroll
@@ -618,15 +664,21 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
     AT* out = reinterpret_cast<AT*>(a.out);
     const VT* vals = reinterpret_cast<const VT*>(a.vals);
     const FT* fin = reinterpret_cast<const FT*>(a.f[inner]);
+    // 32-bit offsets: every factor holds fewer than 2^32 bytes (host-checked)
+    constexpr bool BUF = sizeof(IT) == 4;
+    __amdgpu_buffer_rsrc_t rs[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) rs[m] = factor_rsrc(a.f[m]);
+    const __amdgpu_buffer_rsrc_t rs_in = factor_rsrc(fin);
     for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
         const int col = cb + sub * C;
         AT acc[C], facc[C], scale[C];
         #pragma unroll
         for (int cc = 0; cc < C; ++cc) acc[cc] = facc[cc] = scale[cc] = 0;
-        uint64_t cur = 0;
+        IT cur = 0;
         bool have = false;
         auto flush = [&]() {
-            AT* o = out + cur * (uint64_t)R + col;
+            AT* o = out + (uint64_t)cur * R + col;
             #pragma unroll
             for (int cc = 0; cc < C; ++cc)
                 if (EXACT || col + cc < R) atomic_add_global(o + cc, acc[cc]);
@@ -684,7 +736,10 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
                     const IT iq = __shfl(ioff, src);
                     AT x[C];
                     if (ok) {
-                        gather_row<FT, C, EXACT>(fin + (uint64_t)iq + col, col, R, x);
+                        if (BUF && (EXACT || col + C <= R))
+                            load_row_buf<FT, C>(rs_in, ((uint32_t)iq + col) * (uint32_t)sizeof(FT), x);
+                        else
+                            gather_row<FT, C, EXACT>(fin + (uint64_t)iq + col, col, R, x);
                     } else {
                         #pragma unroll
                         for (int cc = 0; cc < C; ++cc) x[cc] = 0;
@@ -699,8 +754,11 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
                         if (m == mode || m == inner) continue;
                         const IT om = __shfl(off[m], src);
                         if (ok && fs[b]) {
-                            gather_row<FT, C, EXACT>(reinterpret_cast<const FT*>(a.f[m]) + (uint64_t)om + col, col,
-                                                     R, x);
+                            if (BUF && (EXACT || col + C <= R))
+                                load_row_buf<FT, C>(rs[m], ((uint32_t)om + col) * (uint32_t)sizeof(FT), x);
+                            else
+                                gather_row<FT, C, EXACT>(reinterpret_cast<const FT*>(a.f[m]) + (uint64_t)om + col,
+                                                         col, R, x);
                             #pragma unroll
                             for (int cc = 0; cc < C; ++cc) po[b][cc] *= x[cc];
                         }
@@ -716,7 +774,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
                                 acc[cc] += facc[cc] * scale[cc];
                                 facc[cc] = 0;
                             }
-                            if ((uint64_t)rr[b] != cur) {
+                            if (rr[b] != cur) {
                                 flush();
                                 #pragma unroll
                                 for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
@@ -736,7 +794,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
         for (int cc = 0; cc < C; ++cc) acc[cc] += facc[cc] * scale[cc];
         // the slots' last rows: segmented scan over the slots, as in k_mttkrp_sorted
         const int below = lane - L, above = lane + L;
-        const uint64_t prev_cur = __shfl(cur, below < 0 ? lane : below);
+        const IT prev_cur = __shfl(cur, below < 0 ? lane : below);
         const bool prev_have = __shfl(have, below < 0 ? lane : below);
         bool head = slot == 0 || !have || !prev_have || prev_cur != cur;
         #pragma unroll
@@ -754,7 +812,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
                 head = head || pf;
             }
         }
-        const uint64_t next_cur = __shfl(cur, above >= WAVE ? lane : above);
+        const IT next_cur = __shfl(cur, above >= WAVE ? lane : above);
         const bool next_have = __shfl(have, above >= WAVE ? lane : above);
         const bool tail = slot == S - 1 || !next_have || next_cur != cur;
         if (have && tail) flush();
