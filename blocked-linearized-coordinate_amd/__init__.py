@@ -78,6 +78,8 @@ def lib():
             "blco_host_encode": (C.c_int, [C.c_int, u64p, C.c_uint64, u64p, f64p, C.c_uint64,
                                            C.POINTER(vp)]),
             "blco_host_encode_coo": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+            "blco_device_encode": (C.c_int, [C.c_int, u64p, C.c_uint64, u64p, f64p, C.c_uint64,
+                                             C.POINTER(vp)]),
             "blco_host_info": (C.c_int, [vp, C.POINTER(C.c_int), u64p, C.POINTER(C.c_uint64),
                                          C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
             "blco_host_layout": (C.c_int, [vp, u64p, i32p, u64p, u64p]),
@@ -411,6 +413,22 @@ def encode(coo_or_dims, cidx=None, vals=None, max_block_size=0):
         vals = np.ascontiguousarray(vals, np.float64)
         _check(lib().blco_host_encode(len(dims), dims, cidx.shape[1], cidx.reshape(-1), vals,
                                       max_block_size, C.byref(h)))
+    return BlcoHost(h.value)
+
+
+def encode_device(coo_or_dims, cidx=None, vals=None, max_block_size=0):
+    """gen_blcotensor_host computed on the GPU (blco_device_encode): the same
+    BlcoHost as encode(), bit for bit."""
+    if isinstance(coo_or_dims, Coo):
+        dims, cidx, vals = coo_or_dims.dims, coo_or_dims.cidx, coo_or_dims.vals
+    else:
+        dims = coo_or_dims
+    dims = np.ascontiguousarray(dims, np.uint64)
+    cidx = np.ascontiguousarray(cidx, np.uint64)
+    vals = np.ascontiguousarray(vals, np.float64)
+    h = vp()
+    _check(lib().blco_device_encode(len(dims), dims, cidx.shape[1], cidx.reshape(-1), vals,
+                                    max_block_size, C.byref(h)))
     return BlcoHost(h.value)
 
 

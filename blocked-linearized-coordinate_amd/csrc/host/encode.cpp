@@ -99,6 +99,33 @@ int make_layout(int nmodes, const uint64_t* dims, Layout* L)
 static inline uint64_t pdep64(uint64_t x, uint64_t m) { return _pdep_u64(x, m); }
 static inline uint64_t pext64(uint64_t x, uint64_t m) { return _pext_u64(x, m); }
 
+// Blocks of a BLCO tensor (include/blco.hpp:250-321): runs of equal upper
+// key half (hi[i] for sorted nonzero i; none: one run), each split into
+// pieces of at most mbs nonzeros; block coordinates are the upper half's
+// bits of every mode, shifted above its truncated lower bits.
+void build_blocks(const Layout& L, uint64_t nnz, const uint64_t* hi, uint64_t mbs, blco_host* h)
+{
+    const int N = L.nmodes;
+    uint64_t i = 0;
+    while (i < nnz) {
+        const uint64_t bid = hi ? hi[i] : 0;
+        uint64_t j = i + 1;
+        if (hi)
+            while (j < nnz && hi[j] == bid) ++j;
+        else
+            j = nnz;
+        for (uint64_t s = i; s < j; s += mbs) {
+            const uint64_t cnt = (j - s) < mbs ? (j - s) : mbs;
+            h->block_start.push_back(s);
+            h->block_nnz.push_back(cnt);
+            if (cnt > h->max_block_nnz) h->max_block_nnz = cnt;
+            for (int n = 0; n < N; ++n)
+                h->block_coords.push_back(L.alto_hi[n] ? pext64(bid, L.alto_hi[n]) << L.trunc[n] : 0);
+        }
+        i = j;
+    }
+}
+
 template <typename K, typename P>
 static int encode_impl(const Layout& L, uint64_t nnz, const uint64_t* cidx,
                        const double* vals, uint64_t mbs, blco_host* h)
@@ -146,25 +173,13 @@ static int encode_impl(const Layout& L, uint64_t nnz, const uint64_t* cidx,
         h->coords[i] = c;
         h->vals[i] = vals[perm[i]];
     }
-    // blocks: runs of equal upper key half, split at max_block_size
-    uint64_t i = 0;
-    while (i < nnz) {
-        uint64_t bid = sizeof(K) > 8 ? (uint64_t)((u128)keys[i] >> 64) : 0;
-        uint64_t j = i + 1;
-        if (sizeof(K) > 8)
-            while (j < nnz && (uint64_t)((u128)keys[j] >> 64) == bid) ++j;
-        else
-            j = nnz;
-        for (uint64_t s = i; s < j; s += mbs) {
-            uint64_t cnt = (j - s) < mbs ? (j - s) : mbs;
-            h->block_start.push_back(s);
-            h->block_nnz.push_back(cnt);
-            if (cnt > h->max_block_nnz) h->max_block_nnz = cnt;
-            for (int n = 0; n < N; ++n)
-                h->block_coords.push_back(L.alto_hi[n] ? pext64(bid, L.alto_hi[n]) << L.trunc[n] : 0);
-        }
-        i = j;
+    std::vector<uint64_t> hi;
+    if (sizeof(K) > 8) {
+        hi.resize(nnz);
+        #pragma omp parallel for schedule(static)
+        for (uint64_t i = 0; i < nnz; ++i) hi[i] = (uint64_t)((u128)keys[i] >> 64);
     }
+    build_blocks(L, nnz, sizeof(K) > 8 ? hi.data() : nullptr, mbs, h);
     return BLCO_OK;
 }
 

@@ -89,3 +89,10 @@ struct blco_host {
     std::vector<uint64_t> block_nnz;
     std::vector<uint64_t> block_coords;  // nblocks x nmodes
 };
+
+namespace blco {
+// Blocks of a BLCO tensor from the sorted keys' upper halves (hi[i] of
+// sorted nonzero i, or nullptr for keys of at most 64 bits): runs of equal
+// upper half split into pieces of at most mbs nonzeros (encode.cpp).
+void build_blocks(const Layout& L, uint64_t nnz, const uint64_t* hi, uint64_t mbs, blco_host* h);
+}  // namespace blco

@@ -62,6 +62,17 @@ int blco_memcpy_dtod(void* dst, const void* src, uint64_t bytes);
 int blco_memset_zero(void* ptr, uint64_t bytes);
 int blco_synchronize(void);
 
+/* ------------------------------------------------------- device encoder */
+
+/* gen_blcotensor_host<LIT> (include/blco.hpp:196-331) computed on the GPU:
+ * the same blco_host as blco_host_encode, bit for bit (ALTO keys formed and
+ * radix-sorted on the device, BLCO words re-encoded there; blocks built on
+ * the host from the sorted keys' upper halves).  Arguments as for
+ * blco_host_encode: host arrays, cidx mode-major (nmodes x nnz). */
+int blco_device_encode(int nmodes, const uint64_t* dims, uint64_t nnz,
+                       const uint64_t* cidx, const double* vals,
+                       uint64_t max_block_size, blco_host** out);
+
 /* ---------------------------------------------------------- device tensor */
 
 typedef struct blco_dev blco_dev;
