@@ -196,9 +196,14 @@ extern "C" int blco_device_encode(int nmodes, const uint64_t* dims, uint64_t nnz
     if (t2 > tmp_bytes) tmp_bytes = t2;
     unsigned char* tmp;
     if ((rc = B.alloc(&tmp, tmp_bytes ? tmp_bytes : 1))) return done(rc);
-    if (hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, lo, lo2, idx, idx2, nnz, 0, wide ? 64 : L.total_bits) !=
-        hipSuccess)
+    if (L.total_bits == 0) {  // every mode has length 1: nothing to sort
+        if (hipMemcpy(lo2, lo, nnz * 8, hipMemcpyDeviceToDevice) != hipSuccess ||
+            hipMemcpy(idx2, idx, nnz * 8, hipMemcpyDeviceToDevice) != hipSuccess)
+            return done(fail(BLCO_EHIP, "device encoder: copy failed"));
+    } else if (hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, lo, lo2, idx, idx2, nnz, 0,
+                                                  wide ? 64 : L.total_bits) != hipSuccess) {
         return done(fail(BLCO_EHIP, "device encoder: sort failed"));
+    }
     const uint64_t* order = nullptr;
     if (wide) {
         k_gather_u64<<<grid_for(nnz), WG>>>(nnz, hi, idx2, hi2, ord);

@@ -17,6 +17,7 @@ SHAPES = {
     "capped65": ((4800000, 1800000, 1800000), 60000, None, 5000),
     "order2": ((5000, 3000), 20000, None, 0),
     "tiny": ((3, 2, 2), 5, None, 2),
+    "ones": ((1, 1, 1), 1, None, 0),
 }
 
 
