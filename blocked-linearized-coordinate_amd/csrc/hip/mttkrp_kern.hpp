@@ -636,8 +636,8 @@ __device__ __forceinline__ void gather_row(const FT* f, int col, int R, typename
 // mode but the inner one, a.inner), so that a fiber -- the nonzeros sharing
 // the output row and all outer indices -- is contiguous.  Per nonzero only
 // the inner factor row is gathered; the outer factor rows are gathered once
-// per fiber, with its first nonzero, and scale the fiber's sum:
-//   out[i] += sum over fibers (prod_outer U_m[i_m]) * (sum v * U_inner[i_inner]).
+// per fiber, with its first nonzero, and scale every inner term of it:
+//   out[i] += sum over fibers (prod_outer U_m[i_m]) * (v * U_inner[i_inner]).
 // Slots, rounds, the prefetch and the flush are those of k_mttkrp_sorted; a
 // lane also marks whether its nonzero starts a fiber (its key differs from
 // the previous position's, or it is the first of the slot's run).
@@ -672,9 +672,9 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
     const __amdgpu_buffer_rsrc_t rs_in = factor_rsrc(fin);
     for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
         const int col = cb + sub * C;
-        AT acc[C], facc[C], scale[C];
+        AT acc[C], scale[C];
         #pragma unroll
-        for (int cc = 0; cc < C; ++cc) acc[cc] = facc[cc] = scale[cc] = 0;
+        for (int cc = 0; cc < C; ++cc) acc[cc] = scale[cc] = 0;
         IT cur = 0;
         bool have = false;
         auto flush = [&]() {
@@ -768,30 +768,23 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
                 for (int b = 0; b < PB; ++b) {
                     if (k0 + b >= nb) break;
                     if (fs[b]) {
-                        if (have) {
+                        if (have && rr[b] != cur) {
+                            flush();
                             #pragma unroll
-                            for (int cc = 0; cc < C; ++cc) {
-                                acc[cc] += facc[cc] * scale[cc];
-                                facc[cc] = 0;
-                            }
-                            if (rr[b] != cur) {
-                                flush();
-                                #pragma unroll
-                                for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
-                            }
+                            for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
                         }
                         #pragma unroll
                         for (int cc = 0; cc < C; ++cc) scale[cc] = po[b][cc];
                         cur = rr[b];
                         have = true;
                     }
+                    // the fiber's outer product scales each inner term as it
+                    // is added (no separate fiber sum: four fewer VGPRs)
                     #pragma unroll
-                    for (int cc = 0; cc < C; ++cc) facc[cc] += pv[b][cc];
+                    for (int cc = 0; cc < C; ++cc) acc[cc] += pv[b][cc] * scale[cc];
                 }
             }
         }
-        #pragma unroll
-        for (int cc = 0; cc < C; ++cc) acc[cc] += facc[cc] * scale[cc];
         // the slots' last rows: segmented scan over the slots, as in k_mttkrp_sorted
         const int below = lane - L, above = lane + L;
         const IT prev_cur = __shfl(cur, below < 0 ? lane : below);
