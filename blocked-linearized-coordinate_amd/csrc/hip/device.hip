@@ -392,8 +392,8 @@ constexpr double XCD_TILE_SLACK = 1.15;
 constexpr double FIBER_MIN_LEN = 1.25;
 // Fiber order is chosen only when the inner factor holds at least this many
 // bytes at the first call's rank and type (smaller ones stay in L2, where
-// the gathers the fibers save are cheap).  The FP64 fiber kernel holds 134
-// VGPRs (3 waves per SIMD against 4 for the plain kernel), so for FP64
+// the gathers the fibers save are cheap).  The FP64 fiber kernel holds 108
+// VGPRs (4 waves per SIMD against 5 for the plain kernel), so for FP64
 // factors it pays only once most inner gathers miss the caches.
 constexpr uint64_t FIBER_MIN_INNER = 2ull << 20;
 constexpr uint64_t FIBER_MIN_INNER_F64 = 64ull << 20;
