@@ -200,10 +200,18 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
         s_x[e] = (e / R == e % R) ? 1.0 : 0.0;
     }
     __syncthreads();
+    // A pivot at or below R * eps * max_i V_ii means V is singular to working
+    // precision: its Cholesky factor would carry pivots of rounding noise
+    // (and the solve factors of that size), so the pseudo-inverse takes over
+    // -- the threshold the Jacobi fallback uses for eigenvalues.  dpotrf's own
+    // test (a pivot <= 0) is the special case of an exactly singular V.
+    double dmax = 0.0;
+    for (int i = 0; i < R; ++i) dmax = fmax(dmax, s_a[i * R + i]);
+    const double floor_piv = R * 2.220446049250313e-16 * dmax;
     bool ok = true;
     for (int j = 0; j < R; ++j) {
         const double ajj = s_a[j * R + j];
-        if (!(ajj > 0.0)) {  // every thread reads the same value: a uniform exit
+        if (!(ajj > floor_piv)) {  // every thread reads the same value: a uniform exit
             ok = false;
             break;
         }

@@ -86,6 +86,53 @@ def test_solve_pinv_fallback(B, O, gpu):
     assert np.allclose(P, np.linalg.pinv(V, hermitian=True), rtol=1e-6, atol=1e-9 * np.abs(P).max())
 
 
+def test_solve_near_singular_uses_pinv(B, O, gpu):
+    """A column equal to another up to 1e-10 makes V singular to working
+    precision: its Cholesky pivot falls below R * eps * max V_ii, so the
+    pseudo-inverse (not a factor of rounding noise) is applied."""
+    rng = np.random.default_rng(9)
+    R = 8
+    dims = (60, 40, 30)
+    U = [rng.random((d, R)) for d in dims]
+    for u in U:
+        u[:, 5] = u[:, 2] * (1 + 1e-10)
+    grams = [u.T @ u for u in U]
+    V = grams[1] * grams[2]
+    M = rng.random((60, R))
+    Gd = [dev_array(B, g) for g in grams]
+    Md = dev_array(B, M)
+    Ud = B.DeviceArray(M.shape)
+    fb = B.solve(Gd, 0, R, Md, Ud, 0, 60)
+    assert fb == 1
+    want = M @ O.pinv_sym(V)
+    assert np.linalg.norm(Ud.numpy() - want) <= 1e-6 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cpals_degenerate_tensor_stays_finite(B, gpu, precision):
+    """A 4-mode tensor with 27k nonzeros over modes of up to 33683 rows
+    (most rows empty) at rank 16: its normal equations are singular to
+    working precision, where Cholesky factors of rounding noise made the
+    iterates blow up.  With the pivot floor every kernel and precision
+    follows the same smooth trajectory."""
+    dims = (3407, 2410, 6035, 33683)
+    t = B.Coo.generate(dims, 27128, gen_rank=8, value_kind=0, seed=1003026)
+    h = B.encode(t)
+    U0 = B.init_factors(1003026, dims, 16)
+    finals = []
+    for kernel in (B.K_AUTO, B.K_BATCHED, B.K_SELECT):
+        dev = B.DeviceTensor(h, value_dtype=0 if precision == 0 else 1)
+        s = B.CpAls(dev, U0, 16, precision=B.F64 if precision == 0 else B.F32, kernel=kernel)
+        fits = s.iterate(12)
+        U, lam = s.get()
+        s.free()
+        dev.free()
+        assert len(fits) == 12 and np.all(np.isfinite(fits)), (kernel, fits)
+        assert all(np.all(np.isfinite(u)) for u in U) and np.all(np.isfinite(lam))
+        finals.append(fits[-1])
+    assert max(finals) - min(finals) <= 1e-5, finals
+
+
 @pytest.mark.parametrize("which", [0, 1])
 @pytest.mark.parametrize("dtype", [0, 1])
 def test_colnorm(B, O, gpu, which, dtype):
