@@ -128,6 +128,26 @@ def test_fiber_order_without_fiber_kernel(B, O, gpu, tensors, monkeypatch):
         check(B, O, tensors, name, 1, B.K_SORTED, 16)
 
 
+@pytest.mark.parametrize("fibers", ["0", "1"])
+def test_row_ordered_64bit_offsets(B, O, gpu, fibers, monkeypatch):
+    """A rank-64 FP64 factor of 8.5M rows holds more than 4 GiB, so kernel 14
+    (row-ordered and fiber forms) runs its 64-bit-offset variants, which
+    gather through flat addresses instead of 32-bit buffer offsets."""
+    monkeypatch.setenv("BLCO_FIBERS", fibers)
+    dims, R = (8_500_000, 300, 200), 64
+    assert dims[0] * R * 8 > 1 << 32
+    t = B.Coo.generate(dims, 50000, gen_rank=4, value_kind=1, seed=64, topup=False)
+    at = O.Alto(dims, t.cidx, t.vals, nprtn=8)
+    dev = B.DeviceTensor(B.encode(t))
+    rng = np.random.default_rng(64)
+    U = [rng.random((d, R)) for d in dims]
+    for m in range(3):
+        print("64-bit offsets: fibers", fibers, "mode", m, flush=True)
+        e = rel(run_mttkrp(B, dev, U, m, 0, B.K_SORTED), at.mttkrp(U, m))
+        assert e <= TOL[0], (m, e)
+    dev.free()
+
+
 @pytest.mark.parametrize("nprtn", [1, 4, 17])
 def test_lvl3_short_modes(B, O, gpu, nprtn):
     dims = (60, 90, 5000)
