@@ -68,15 +68,25 @@ void from_dtype(const void* src, double* dst, uint64_t n, int dt)
     }
 }
 
+int enqueue_factorize(blco_cpals_state* s, int n, hipStream_t st)
+{
+    GramSet gs;
+    for (int m = 0; m < s->N; ++m) gs.g[m] = s->G[m];
+    return launch_factorize(gs, s->N, n, s->R, s->fac, s->work, s->status, st);
+}
+
 int enqueue_step(blco_cpals_state* s, int n, int which_norm, hipStream_t st)
+{
+    int rc = enqueue_factorize(s, n, st);
+    return rc ? rc : enqueue_update(s, n, which_norm, st);
+}
+
+int enqueue_update(blco_cpals_state* s, int n, int which_norm, hipStream_t st)
 {
     const int N = s->N, R = s->R;
     const uint64_t I = s->d->dims[n];
-    GramSet gs;
-    for (int m = 0; m < N; ++m) gs.g[m] = s->G[m];
     const int last = s->order[N - 1];
-    int rc = launch_factorize(gs, N, n, R, s->fac, s->work, s->status, st);
-    if (!rc) rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, st);
+    int rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, st);
     if (!rc) rc = launch_lambda(s->stats, (int)solve_parts(I), R, which_norm, s->lambda, s->inv, st);
     if (!rc) rc = launch_gram(s->U[n], s->ft, I, R, s->inv, n == last ? s->Mbuf : nullptr, s->gram_part,
                               s->G[n], s->ip, st);
@@ -97,14 +107,21 @@ static int enqueue_sweep(blco_cpals_state* s, int which_norm, bool timed)
     const int N = s->N, R = s->R;
     for (int k = 0; k < N; ++k) {
         const int n = s->order[k];
-        int rc = launch_fill(s->Mbuf, d->dims[n] * (uint64_t)R, s->mt, 0.0, s->st);
+        // V_n's factorisation on the side stream, beside the MTTKRP
+        HIP_TRY(hipEventRecord(s->fork[n], s->st));
+        HIP_TRY(hipStreamWaitEvent(s->side, s->fork[n], 0));
+        int rc = enqueue_factorize(s, n, s->side);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(s->join[n], s->side));
+        rc = launch_fill(s->Mbuf, d->dims[n] * (uint64_t)R, s->mt, 0.0, s->st);
         if (rc) return rc;
         if (timed) HIP_TRY(hipEventRecord(s->ev0[n], s->st));
         rc = mttkrp(d, n, R, s->ft, s->U, s->Mbuf, s->o.kernel, s->o.nprtn, s->o.thread_cf, s->st,
                     &s->kern[n]);
         if (rc) return rc;
         if (timed) HIP_TRY(hipEventRecord(s->ev1[n], s->st));
-        rc = enqueue_step(s, n, which_norm, s->st);
+        HIP_TRY(hipStreamWaitEvent(s->st, s->join[n], 0));
+        rc = enqueue_update(s, n, which_norm, s->st);
         if (rc) return rc;
     }
     return enqueue_fit(s, s->st);
@@ -124,6 +141,7 @@ void blco_cpals_free(blco_cpals_state* s)
 {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
+    if (s->side) (void)hipStreamSynchronize(s->side);
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (int m = 0; m < BLCO_MAX_MODES; ++m) {
@@ -131,7 +149,10 @@ void blco_cpals_free(blco_cpals_state* s)
         if (s->G[m]) (void)hipFree(s->G[m]);
         if (s->ev0[m]) (void)hipEventDestroy(s->ev0[m]);
         if (s->ev1[m]) (void)hipEventDestroy(s->ev1[m]);
+        if (s->fork[m]) (void)hipEventDestroy(s->fork[m]);
+        if (s->join[m]) (void)hipEventDestroy(s->join[m]);
     }
+    if (s->side) (void)hipStreamDestroy(s->side);
     void* bufs[] = {s->Mbuf, s->lambda, s->inv, s->fac, s->work, s->status, s->gram_part,
                     s->stats, s->ip, s->fitbuf, s->norm_part};
     for (void* p : bufs)
@@ -188,9 +209,13 @@ int blco_cpals_create(blco_dev* d, const blco_cpals_opts* o, const double* const
         to_dtype(U[m], stage.data(), n, s->ft);
         if (hipMemcpy(s->U[m], stage.data(), n * fsz, hipMemcpyHostToDevice) != hipSuccess)
             return bad(fail(BLCO_EHIP, "factor upload failed"));
-        if (hipEventCreate(&s->ev0[m]) != hipSuccess || hipEventCreate(&s->ev1[m]) != hipSuccess)
+        if (hipEventCreate(&s->ev0[m]) != hipSuccess || hipEventCreate(&s->ev1[m]) != hipSuccess ||
+            hipEventCreateWithFlags(&s->fork[m], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->join[m], hipEventDisableTiming) != hipSuccess)
             return bad(fail(BLCO_EHIP, "event creation failed"));
     }
+    if (hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess)
+        return bad(fail(BLCO_EHIP, "stream creation failed"));
     const int gparts = GRAM_MAX_PARTS;  // upper bound of gram_parts() for any row count
     uint32_t sparts = solve_parts(maxI);
     const int nnorm = 1024;

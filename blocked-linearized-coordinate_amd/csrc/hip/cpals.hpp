@@ -25,6 +25,10 @@ struct blco_cpals_state {
     int order[BLCO_MAX_MODES] = {};
     int kern[BLCO_MAX_MODES] = {};
     hipEvent_t ev0[BLCO_MAX_MODES] = {}, ev1[BLCO_MAX_MODES] = {};
+    // side stream of the sweep: mode n's R x R factorisation needs only the
+    // other modes' Grams, so it runs there while mode n's MTTKRP runs on st
+    hipStream_t side = nullptr;
+    hipEvent_t fork[BLCO_MAX_MODES] = {}, join[BLCO_MAX_MODES] = {};
     hipEvent_t it0 = nullptr, it1 = nullptr;
     double mttkrp_ms[BLCO_MAX_MODES] = {};
     double iter_ms = 0.0;
@@ -45,6 +49,10 @@ void from_dtype(const void* src, double* dst, uint64_t n, int dt);
 // normalisation and the new Gram (plus the fit's inner product when n is the
 // last mode of the sweep), on stream st.
 int enqueue_step(blco_cpals_state* s, int n, int which_norm, hipStream_t st);
+// The two halves of enqueue_step: the factorisation of V_n (needs the other
+// modes' Grams only) and the rest (needs it and the MTTKRP).
+int enqueue_factorize(blco_cpals_state* s, int n, hipStream_t st);
+int enqueue_update(blco_cpals_state* s, int n, int which_norm, hipStream_t st);
 // The fit of the current model into s->fitbuf, on stream st.
 int enqueue_fit(blco_cpals_state* s, hipStream_t st);
 
