@@ -2,7 +2,7 @@
 the oracle's restatement of cpd_alto and against closed forms.
 
 Tolerances: Gram / solve / norm / fit pieces in FP64 to 1e-11 relative; the
-FP64 CP-ALS fit trajectory to 1e-8 absolute per iteration (north-star bar);
+FP64 CP-ALS fit trajectory to 1e-8 absolute per iteration (the plain form's bar);
 the FP32 / BF16 forms' final fit to 1e-4 / 5e-3 absolute of the FP64 oracle.
 """
 import numpy as np
