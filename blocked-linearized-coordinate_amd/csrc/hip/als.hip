@@ -13,10 +13,12 @@
 //              pass scales U by 1/lambda first (normalisation) and forms the
 //              fit inner products sum_i U_ir M_ir.
 //   factorize  one workgroup: V = prod_{m != n} G_m (Hadamard), lower
-//              Cholesky in LAPACK dpotf2 order in LDS; when a pivot is not
-//              positive, cyclic Jacobi eigendecomposition of V and the
-//              pseudo-inverse V^+ = Q diag(1/w) Q^T with tolerance
-//              R * ulp(max |w|) (src/cpd_gpu.cu:487-489).
+//              Cholesky in LAPACK dpotf2 order in LDS; when a pivot falls to
+//              R * eps * max V_ii or below, cyclic Jacobi eigendecomposition
+//              of V and the pseudo-inverse V^+ = Q diag(1/w) Q^T with
+//              tolerance R * ulp(max |w|) (src/cpd_gpu.cu:487-489), where eps
+//              and ulp are those of the precision the Grams were accumulated
+//              in (f64, or f32 for the f32 / bf16 factor forms).
 //              On success the explicit inverse V^{-1} = L^{-T} L^{-1} is
 //              formed (so both outcomes leave one R x R matrix F).
 //   solve rows one thread per row: u = F m with F broadcast from LDS; the
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(WG) void k_gram_reduce(const double* __restrict__ p
 // is formed row by row with the same parallel pattern, then
 // V^{-1} = L^{-T} L^{-1}.
 __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mode, int R,
-                                                  double* __restrict__ fac,
+                                                  double eps, double* __restrict__ fac,
                                                   double* __restrict__ work,
                                                   int* __restrict__ status)
 {
@@ -201,13 +203,17 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
     }
     __syncthreads();
     // A pivot at or below R * eps * max_i V_ii means V is singular to working
-    // precision: its Cholesky factor would carry pivots of rounding noise
-    // (and the solve factors of that size), so the pseudo-inverse takes over
-    // -- the threshold the Jacobi fallback uses for eigenvalues.  dpotrf's own
-    // test (a pivot <= 0) is the special case of an exactly singular V.
+    // precision its Grams carry: its Cholesky factor would carry pivots of
+    // rounding noise (and the solve factors of that size), so the
+    // pseudo-inverse takes over -- the threshold the Jacobi fallback uses for
+    // eigenvalues.  eps is that of the Grams' accumulation (2^-52 for f64
+    // factors, 2^-23 for f32 / bf16 ones: an f32 V that is singular to f32
+    // precision clears an f64 floor or not depending on the summation order).
+    // dpotrf's own test (a pivot <= 0) is the special case of an exactly
+    // singular V.
     double dmax = 0.0;
     for (int i = 0; i < R; ++i) dmax = fmax(dmax, s_a[i * R + i]);
-    const double floor_piv = R * 2.220446049250313e-16 * dmax;
+    const double floor_piv = R * eps * dmax;
     bool ok = true;
     for (int j = 0; j < R; ++j) {
         const double ajj = s_a[j * R + j];
@@ -255,7 +261,10 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
     double* Q = work + R * R;  // R x R eigenvectors (columns)
     double* wv = work + 2 * R * R;
     for (int e = tid; e < R * R; e += WG) Q[e] = (e / R == e % R) ? 1.0 : 0.0;
-    __shared__ int s_done;
+    // s_done (sweep converged) and s_skip (one pair's rotation skipped) are
+    // separate flags, and every thread has read s_done before thread 0 can
+    // write either again, so no wave acts on a flag of another step.
+    __shared__ int s_done, s_skip;
     __shared__ double s_c, s_s;
     __syncthreads();
     for (int sweep = 0; sweep < 60; ++sweep) {
@@ -270,23 +279,25 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
             s_done = (off <= 1e-30 * diag || off == 0.0);
         }
         __syncthreads();
-        if (s_done) break;
+        const bool converged = s_done != 0;
+        __syncthreads();
+        if (converged) break;
         for (int p = 0; p < R - 1; ++p)
             for (int qq = p + 1; qq < R; ++qq) {
                 if (tid == 0) {
                     double apq = A[p * R + qq];
                     if (apq == 0.0) {
-                        s_c = 1.0; s_s = 0.0; s_done = 2;
+                        s_c = 1.0; s_s = 0.0; s_skip = 1;
                     } else {
                         double app = A[p * R + p], aqq = A[qq * R + qq];
                         double tau = (aqq - app) / (2.0 * apq);
                         double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         double c = 1.0 / sqrt(1.0 + t * t);
-                        s_c = c; s_s = t * c; s_done = 0;
+                        s_c = c; s_s = t * c; s_skip = 0;
                     }
                 }
                 __syncthreads();
-                if (s_done == 2) { __syncthreads(); continue; }
+                if (s_skip) { __syncthreads(); continue; }
                 const double c = s_c, s = s_s;
                 for (int k = tid; k < R; k += WG) {
                     double akp = A[k * R + p], akq = A[k * R + qq];
@@ -310,7 +321,9 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
     if (tid == 0) {
         double emax = 0.0;
         for (int i = 0; i < R; ++i) emax = fmax(emax, fabs(A[i * R + i]));
-        double tol = R * (nextafter(emax, emax + 1.0) - emax);
+        // R * ulp(emax) in the Grams' precision; with eps = 2^-52 this is
+        // R * (nextafter(emax, +inf) - emax) of src/cpd_gpu.cu:487-489
+        double tol = emax > 0.0 ? R * ldexp(eps, ilogb(emax)) : 0.0;
         for (int i = 0; i < R; ++i) {
             double x = A[i * R + i];
             wv[i] = (fabs(x) > tol) ? 1.0 / x : 0.0;
@@ -565,13 +578,19 @@ int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scal
     return BLCO_OK;
 }
 
-int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double* fac, double* work,
-                     int* status, hipStream_t st)
+double gram_eps(int factor_dtype)
 {
+    return factor_dtype == BLCO_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+}
+
+int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps, double* fac,
+                     double* work, int* status, hipStream_t st)
+{
+    if (R < 1 || R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] in the factorisation", R);
     size_t lds = sizeof(double) * 2 * (size_t)R * R;
     if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute((const void*)k_factorize,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_factorize, dim3(1), dim3(WG), lds, st, gs, nmodes, mode, R, fac, work, status);
+    hipLaunchKernelGGL(k_factorize, dim3(1), dim3(WG), lds, st, gs, nmodes, mode, R, eps, fac, work, status);
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }

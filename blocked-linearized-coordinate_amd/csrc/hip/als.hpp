@@ -20,9 +20,13 @@ int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg);
 // gram_parts(rows) * (R*R + R) doubles.
 int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
                 double* part, double* G, double* ip, hipStream_t st);
-// fac: R x R (V^{-1}, or V^+ with status 1); work: 2 R^2 + R doubles.
-int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double* fac, double* work,
-                     int* status, hipStream_t st);
+// Machine epsilon of the precision the Grams of factor_dtype factors are
+// accumulated in (f64 for f64 factors, f32 for f32 and bf16 ones).
+double gram_eps(int factor_dtype);
+// fac: R x R (V^{-1}, or V^+ with status 1); work: 2 R^2 + R doubles.  eps
+// scales the Cholesky pivot floor and the pseudo-inverse tolerance.
+int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps, double* fac,
+                     double* work, int* status, hipStream_t st);
 uint32_t solve_parts(uint64_t rows);
 // stats: solve_parts(rows) * R doubles.
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,

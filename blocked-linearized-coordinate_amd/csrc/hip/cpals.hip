@@ -72,7 +72,7 @@ int enqueue_factorize(blco_cpals_state* s, int n, hipStream_t st)
 {
     GramSet gs;
     for (int m = 0; m < s->N; ++m) gs.g[m] = s->G[m];
-    return launch_factorize(gs, s->N, n, s->R, s->fac, s->work, s->status, st);
+    return launch_factorize(gs, s->N, n, s->R, gram_eps(s->ft), s->fac, s->work, s->status, st);
 }
 
 int enqueue_step(blco_cpals_state* s, int n, int which_norm, hipStream_t st)

@@ -1327,25 +1327,36 @@ int blco_vecsum(double* x, uint64_t n, void* stream)
 int blco_solve(double* const* grams, int nmodes, int mode, int rank, const void* M, void* U, int dtype,
                uint64_t rows, int* used_fallback, void* stream)
 {
-    if (!grams || !M || !U || nmodes < 2 || nmodes > BLCO_MAX_MODES || mode < 0 || mode >= nmodes ||
-        rank < 1 || rank > 128)
+    if (!grams || !M || !U || nmodes < 2 || nmodes > BLCO_MAX_MODES || mode < 0 || mode >= nmodes)
         return fail(BLCO_EINVAL, "bad solve arguments");
+    if (rank < 1 || rank > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] in the solve", rank);
+    if (dtype < BLCO_F64 || dtype > BLCO_BF16) return fail(BLCO_EINVAL, "factor dtype %d", dtype);
     hipStream_t st = (hipStream_t)stream;
     GramSet gs;
     for (int m = 0; m < nmodes; ++m) gs.g[m] = grams[m];
     double *fac = nullptr, *work = nullptr, *stats = nullptr;
     int* status = nullptr;
     uint32_t np = solve_parts(rows);
-    HIP_TRY(hipMalloc(&fac, sizeof(double) * rank * rank));
-    HIP_TRY(hipMalloc(&work, sizeof(double) * (2 * rank * rank + rank)));
-    HIP_TRY(hipMalloc(&status, sizeof(int)));
-    HIP_TRY(hipMalloc(&stats, sizeof(double) * (size_t)(np ? np : 1) * rank));
-    int rc = launch_factorize(gs, nmodes, mode, rank, fac, work, status, st);
+    auto release = [&]() {
+        if (fac) (void)hipFree(fac);
+        if (work) (void)hipFree(work);
+        if (status) (void)hipFree(status);
+        if (stats) (void)hipFree(stats);
+    };
+    if (hipMalloc(&fac, sizeof(double) * rank * rank) != hipSuccess ||
+        hipMalloc(&work, sizeof(double) * (2 * rank * rank + rank)) != hipSuccess ||
+        hipMalloc(&status, sizeof(int)) != hipSuccess ||
+        hipMalloc(&stats, sizeof(double) * (size_t)(np ? np : 1) * rank) != hipSuccess) {
+        release();
+        return fail(BLCO_ENOMEM, "solve work allocation failed");
+    }
+    // the Grams of f32 / bf16 factors carry f32 precision (gram_eps)
+    int rc = launch_factorize(gs, nmodes, mode, rank, gram_eps(dtype), fac, work, status, st);
     if (!rc) rc = launch_solve_rows(M, U, dtype, rows, rank, fac, 0, stats, st);
     int h_status = 0;
     hipError_t e = hipMemcpyAsync(&h_status, status, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(fac); (void)hipFree(work); (void)hipFree(status); (void)hipFree(stats);
+    release();
     if (rc) return rc;
     HIP_TRY(e);
     if (used_fallback) *used_fallback = h_status;

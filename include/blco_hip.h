@@ -134,11 +134,14 @@ int blco_vecsum(double* x, uint64_t n, void* stream);
  * reference's Cholesky / minimum-norm fallback (include/cpd.hpp:547-648):
  * V = Hadamard of grams[m] for m != mode (nmodes device pointers, f64
  * rank x rank); U = M V^{-1} row by row (Cholesky, forward/backward
- * substitution), or M V^+ when V is not positive definite (symmetric
- * pseudo-inverse, tolerance rank * ulp(max |eig|)).  M (rows x rank, f64 or
- * f32 per dtype) is read, U (same dtype) written.  *used_fallback (host, may
- * be NULL) receives 1 when the pseudo-inverse was used (this reads back one
- * flag and synchronises the stream). */
+ * substitution), or M V^+ when a Cholesky pivot falls to
+ * rank * eps * max V_ii or below (symmetric pseudo-inverse, tolerance
+ * rank * ulp(max |eig|)); eps and ulp are f64's for dtype BLCO_F64 and f32's
+ * for BLCO_F32 / BLCO_BF16 factors, whose Grams carry f32 precision.  M
+ * (rows x rank, f64 for dtype BLCO_F64, else f32) is read, U (dtype) written.
+ * rank <= 64.  *used_fallback (host, may be NULL) receives 1 when the
+ * pseudo-inverse was used (this reads back one flag and synchronises the
+ * stream). */
 int blco_solve(double* const* grams, int nmodes, int mode, int rank,
                const void* M, void* U, int dtype, uint64_t rows,
                int* used_fallback, void* stream);

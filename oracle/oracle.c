@@ -40,6 +40,10 @@
  *                         densely on tiny tensors; an exact low-rank tensor
  *                         converges to fit -> 1; one ALS sweep equals a numpy
  *                         normal-equation restatement.
+ *   orc_cpd_alto_ex       pivot_eps = 0 is orc_cpd_alto bit for bit; with a
+ *                         floor, a planted near-duplicate column switches the
+ *                         solve to the pseudo-inverse and one sweep equals
+ *                         the numpy restatement with numpy.linalg.pinv.
  * The reference cpd_alto / pseudo_inverse / SYRK cannot be compiled here (they
  * need cblas.h / lapacke.h), so the ALS step is pinned by the closed forms
  * above, not by the reference binary.
@@ -557,14 +561,15 @@ int orc_alto_mttkrp(orc_alto* at, int mode, int rank, double* const* U,
 /* ------------------------------------------------- normal equations */
 
 /* Unblocked lower Cholesky in LAPACK dpotf2 order (the reference calls
- * dpotrf_ with uplo 'L', cpd.hpp:594). */
-int orc_cholesky(int R, const double* V, double* L)
+ * dpotrf_ with uplo 'L', cpd.hpp:594).  A pivot at or below `floor` fails
+ * (dpotrf: floor 0). */
+static int cholesky_floor(int R, const double* V, double* L, double floor)
 {
     for (int i = 0; i < R * R; ++i) L[i] = 0.0;
     for (int j = 0; j < R; ++j) {
         double ajj = V[j * R + j];
         for (int k = 0; k < j; ++k) ajj -= L[j * R + k] * L[j * R + k];
-        if (!(ajj > 0.0)) return j + 1;
+        if (!(ajj > floor)) return j + 1;
         ajj = sqrt(ajj);
         L[j * R + j] = ajj;
         for (int i = j + 1; i < R; ++i) {
@@ -575,6 +580,8 @@ int orc_cholesky(int R, const double* V, double* L)
     }
     return 0;
 }
+
+int orc_cholesky(int R, const double* V, double* L) { return cholesky_floor(R, V, L, 0.0); }
 
 /* dpotrs_ with one right-hand side per row of B, cpd.hpp:597-599. */
 void orc_cholesky_solve_rows(int R, const double* L, uint64_t nrows, double* B)
@@ -637,7 +644,10 @@ static void jacobi_eig(int R, double* A, double* Q)
     }
 }
 
-void orc_pinv_sym(int R, const double* V, double* P)
+/* tol = R * ulp(emax) in the precision of machine epsilon eps: for
+ * eps = DBL_EPSILON, R * (nextafter(emax, emax+1) - emax) of
+ * src/cpd_gpu.cu:487-489. */
+static void pinv_sym_eps(int R, const double* V, double* P, double eps)
 {
     double* A = (double*)malloc(sizeof(double) * R * R);
     double* Q = (double*)malloc(sizeof(double) * R * R);
@@ -649,8 +659,8 @@ void orc_pinv_sym(int R, const double* V, double* P)
         w[i] = A[i * R + i];
         if (fabs(w[i]) > emax) emax = fabs(w[i]);
     }
-    /* tol = n * (nextafter(s, s+1) - s), src/cpd_gpu.cu:487-489 */
-    double tol = R * (nextafter(emax, emax + 1.0) - emax);
+    double tol = (eps == DBL_EPSILON) ? R * (nextafter(emax, emax + 1.0) - emax)
+                 : (emax > 0.0 ? R * ldexp(eps, ilogb(emax)) : 0.0);
     for (int i = 0; i < R; ++i) w[i] = (fabs(w[i]) > tol) ? 1.0 / w[i] : 0.0;
     for (int i = 0; i < R; ++i)
         for (int j = 0; j < R; ++j) {
@@ -662,6 +672,8 @@ void orc_pinv_sym(int R, const double* V, double* P)
     free(Q);
     free(w);
 }
+
+void orc_pinv_sym(int R, const double* V, double* P) { pinv_sym_eps(R, V, P, DBL_EPSILON); }
 
 /* KruskalModelNorm, src/kruskal_model.cpp:142-259.  A zero L2 column is
  * left at zero with lambda 0 (the reference divides by zero on the CPU and
@@ -736,10 +748,15 @@ double orc_cpd_fit(int nmodes, uint64_t last_rows, int rank,
     return 1.0 - res / sqrt(normX2);
 }
 
-/* cpd_alto, include/cpd.hpp:44-187 */
-int orc_cpd_alto(orc_alto* at, int rank, double** U, double* lambda,
-                 int max_iters, double epsilon, double* fit_trace,
-                 double* iter_seconds, double* mttkrp_seconds, int verbose)
+/* cpd_alto, include/cpd.hpp:44-187.  pivot_eps = 0: the reference's solve
+ * (dpotrf fails on a pivot <= 0, then the pseudo-inverse with tolerance
+ * R * ulp_f64); pivot_eps > 0: a pivot at or below R * pivot_eps * max V_ii
+ * also fails and the pseudo-inverse tolerance is R * ulp in the precision of
+ * pivot_eps (the rule of the GPU library's factorisation). */
+int orc_cpd_alto_ex(orc_alto* at, int rank, double** U, double* lambda,
+                    int max_iters, double epsilon, double* fit_trace,
+                    double* iter_seconds, double* mttkrp_seconds, int verbose,
+                    double pivot_eps)
 {
     int N = at->nmode, R = rank;
     double normX2 = 0.0;
@@ -773,11 +790,17 @@ int orc_cpd_alto(orc_alto* at, int rank, double** U, double* lambda,
             for (int m = 0; m < N; ++m)
                 if (m != j)
                     for (int k = 0; k < R * R; ++k) V[k] *= grams[m][k];
-            if (orc_cholesky(R, V, L) == 0) {
+            double floor = 0.0;
+            if (pivot_eps > 0.0) {
+                double dmax = 0.0;
+                for (int k = 0; k < R; ++k) dmax = fmax(dmax, V[k * R + k]);
+                floor = R * pivot_eps * dmax;
+            }
+            if (cholesky_floor(R, V, L, floor) == 0) {
                 orc_cholesky_solve_rows(R, L, I, U[j]);
             } else {
                 /* rank-deficient fallback: minimum-norm solution V^+ b */
-                orc_pinv_sym(R, V, P);
+                pinv_sym_eps(R, V, P, pivot_eps > 0.0 ? pivot_eps : DBL_EPSILON);
                 if (tmp_rows < I) { free(tmp); tmp = (double*)malloc(sizeof(double) * I * R); tmp_rows = I; }
                 memcpy(tmp, U[j], sizeof(double) * I * R);
                 #pragma omp parallel for schedule(static)
@@ -807,4 +830,12 @@ int orc_cpd_alto(orc_alto* at, int rank, double** U, double* lambda,
     for (int n = 0; n < N; ++n) free(grams[n]);
     free(V); free(L); free(P); free(scratch); free(tmp);
     return done;
+}
+
+int orc_cpd_alto(orc_alto* at, int rank, double** U, double* lambda,
+                 int max_iters, double epsilon, double* fit_trace,
+                 double* iter_seconds, double* mttkrp_seconds, int verbose)
+{
+    return orc_cpd_alto_ex(at, rank, U, lambda, max_iters, epsilon, fit_trace,
+                           iter_seconds, mttkrp_seconds, verbose, 0.0);
 }

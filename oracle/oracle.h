@@ -90,6 +90,15 @@ int orc_cpd_alto(orc_alto* at, int rank, double** U, double* lambda,
                  int max_iters, double epsilon, double* fit_trace,
                  double* iter_seconds, double* mttkrp_seconds, int verbose);
 
+/* orc_cpd_alto with the GPU library's solve rule: pivot_eps > 0 makes a
+ * Cholesky pivot at or below R * pivot_eps * max V_ii fail (as a pivot <= 0
+ * does) and sets the pseudo-inverse tolerance to R * ulp in the precision of
+ * pivot_eps; pivot_eps = 0 is orc_cpd_alto. */
+int orc_cpd_alto_ex(orc_alto* at, int rank, double** U, double* lambda,
+                    int max_iters, double epsilon, double* fit_trace,
+                    double* iter_seconds, double* mttkrp_seconds, int verbose,
+                    double pivot_eps);
+
 /* Normal-equation pieces used by orc_cpd_alto, exported for unit tests. */
 /* Lower Cholesky V = L L^T (LAPACK dpotf2 order).  Returns 0 or j+1 for the
  * first non-positive pivot.  V is row-major R x R; L written to the lower

@@ -50,6 +50,9 @@ def lib():
         L.orc_cpd_alto.restype = C.c_int
         L.orc_cpd_alto.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), f64p,
                                    C.c_int, C.c_double, f64p, f64p, f64p, C.c_int]
+        L.orc_cpd_alto_ex.restype = C.c_int
+        L.orc_cpd_alto_ex.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), f64p,
+                                      C.c_int, C.c_double, f64p, f64p, f64p, C.c_int, C.c_double]
         L.orc_cholesky.restype = C.c_int
         L.orc_cholesky.argtypes = [C.c_int, f64p, f64p]
         L.orc_cholesky_solve_rows.argtypes = [C.c_int, f64p, C.c_uint64, f64p]
@@ -181,16 +184,18 @@ class Alto:
             raise ValueError("orc_alto_mttkrp failed")
         return out
 
-    def cpd(self, U, max_iters, epsilon=0.0, verbose=False):
-        """Runs CP-ALS in place on copies of U; returns (U, lambda, fits, times, mttkrp_times)."""
+    def cpd(self, U, max_iters, epsilon=0.0, verbose=False, pivot_eps=0.0):
+        """Runs CP-ALS in place on copies of U; returns (U, lambda, fits, times, mttkrp_times).
+        pivot_eps > 0 applies the GPU library's pivot floor R * pivot_eps * max V_ii
+        and pseudo-inverse tolerance; 0 is the reference's solve rule."""
         R = U[0].shape[1]
         Uc = [np.array(u, dtype=np.float64, order="C", copy=True) for u in U]
         lam = np.zeros(R, np.float64)
         fits = np.zeros(max(max_iters, 1), np.float64)
         its = np.zeros(max(max_iters, 1), np.float64)
         mts = np.zeros(max(max_iters, 1), np.float64)
-        n = lib().orc_cpd_alto(self.h, R, _ptrs(Uc), lam, max_iters, epsilon, fits, its,
-                               mts, 1 if verbose else 0)
+        n = lib().orc_cpd_alto_ex(self.h, R, _ptrs(Uc), lam, max_iters, epsilon, fits, its,
+                                  mts, 1 if verbose else 0, float(pivot_eps))
         return Uc, lam, fits[:n].copy(), its[:n].copy(), mts[:n].copy()
 
 

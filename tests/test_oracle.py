@@ -361,3 +361,55 @@ def test_alto_mttkrp_reused_at_growing_ranks(O):
         for m in range(3):
             want = O.mttkrp_coo(c, v, U, m)
             assert np.linalg.norm(at.mttkrp(U, m) - want) <= 1e-13 * np.linalg.norm(want)
+
+
+def test_cpd_pivot_floor_sweep_is_minimum_norm(O):
+    """With the pivot floor (pivot_eps = 2^-52), a V that is singular to
+    working precision (a factor column equal to another up to 1e-10 in every
+    mode) is solved by the pseudo-inverse: one sweep equals the numpy
+    restatement with numpy.linalg.pinv (SVD-based, cutoff R * eps * sigma_max)
+    in place of the solve."""
+    rng = np.random.default_rng(38)
+    dims = (14, 11, 9)
+    c, v = random_coo(rng, dims, 500)
+    R = 5
+    U0 = [rng.random((d, R)) for d in dims]
+    for u in U0:
+        u[:, 3] = u[:, 1] * (1 + 1e-10)
+    at = O.Alto(dims, c, v, nprtn=3)
+    U1, lam1, _, _, _ = at.cpd(U0, 1, pivot_eps=2.0 ** -52)
+    X = dense_of(dims, c, v)
+    U = [u.copy() for u in U0]
+    for n in range(3):
+        M = mttkrp_dense(X, U, n)
+        V = np.ones((R, R))
+        for m in range(3):
+            if m != n:
+                V = V * (U[m].T @ U[m])
+        Un = M @ np.linalg.pinv(V, rcond=R * 2.0 ** -52, hermitian=True)
+        lam = np.linalg.norm(Un, axis=0)
+        U[n] = Un / lam
+    for a, b in zip(U1, U):
+        assert np.allclose(a, b, rtol=1e-6, atol=1e-9)
+    assert np.allclose(lam1, lam, rtol=1e-6)
+
+
+def test_cpd_pivot_floor_is_order_independent(B, O):
+    """The degenerate 4-mode tensor of the batched set (seed 1003026, rank
+    16): under the reference's rule (a pivot <= 0 fails) the trajectory
+    depends on the MTTKRP summation order (1 vs 8 partitions) once V becomes
+    singular to working precision; with the pivot floor, in f64 or f32
+    precision, both orders give one trajectory."""
+    dims = (3407, 2410, 6035, 33683)
+    t = B.Coo.generate(dims, 27128, gen_rank=8, value_kind=0, seed=1003026)
+    U0 = B.init_factors(1003026, dims, 16)
+    a1 = O.Alto(dims, t.cidx, t.vals, nprtn=1)
+    a8 = O.Alto(dims, t.cidx, t.vals, nprtn=8)
+    r1 = a1.cpd(U0, 12, 0.0)[2]
+    r8 = a8.cpd(U0, 12, 0.0)[2]
+    assert np.abs(r1 - r8).max() > 1e-3
+    for pe in (2.0 ** -52, 2.0 ** -23):
+        f1 = a1.cpd(U0, 12, 0.0, pivot_eps=pe)[2]
+        f8 = a8.cpd(U0, 12, 0.0, pivot_eps=pe)[2]
+        assert np.abs(f1 - f8).max() <= 1e-10, (pe, f1, f8)
+        assert np.isfinite(f1).all() and abs(f1[-1] - 0.0356149) < 1e-6
