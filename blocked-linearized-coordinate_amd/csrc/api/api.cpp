@@ -289,6 +289,9 @@ blcotensor* gen_blcotensor_host_any(SparseTensor* X, IType max_block_size)
         blco_block* k = gen_block_host(nm, bn[b], B);
         k->idx = B->coords + bs[b];
         k->vals = B->values + bs[b];
+        k->dims = B->modes;
+        k->mode_masks = B->mode_masks;
+        k->mode_pos = B->mode_pos;
         for (int n = 0; n < nm; ++n) k->block_coords[n] = bc[b * nm + n];
         B->blocks[b] = k;
     }
@@ -306,9 +309,34 @@ blcotensor* gen_blcotensor_device(blcotensor* X, IType /*block_count*/, IType MB
     D->modes = new IType[X->N];
     std::memcpy(D->modes, X->modes, sizeof(IType) * X->N);
     D->total_nnz = X->total_nnz;
+    D->max_nnz = X->max_nnz;
     D->block_count = X->block_count;
     D->max_block_size = MB;
+    D->mode_masks = new IType[X->N];
+    D->mode_pos = new int[X->N];
+    std::memcpy(D->mode_masks, X->mode_masks, sizeof(IType) * X->N);
+    std::memcpy(D->mode_pos, X->mode_pos, sizeof(int) * X->N);
     D->on_device = true;
+    // block records of the device tensor: device coordinates (and f64
+    // values) of each resident block, host block coordinates
+    uint64_t* dc = nullptr;
+    void* dv = nullptr;
+    int vdt = 0;
+    die_if(blco_dev_arrays(D->dev, &dc, &dv, &vdt, nullptr), "gen_blcotensor_device");
+    D->blocks = new blco_block*[X->block_count ? X->block_count : 1];
+    uint64_t off = 0;
+    for (IType b = 0; b < X->block_count; ++b) {
+        const blco_block* hb = X->blocks[b];
+        blco_block* k = gen_block_device(X->N, hb->nnz, D);
+        std::memcpy(k->block_coords, hb->block_coords, sizeof(IType) * X->N);
+        k->dims = D->modes;
+        k->mode_masks = D->mode_masks;
+        k->mode_pos = D->mode_pos;
+        if (dc) k->idx = reinterpret_cast<IType*>(dc) + off;
+        if (dv && vdt == BLCO_F64) k->vals = static_cast<FType*>(dv) + off;
+        off += hb->nnz;
+        D->blocks[b] = k;
+    }
     return D;
 }
 
@@ -327,8 +355,13 @@ void delete_blcotensor_host(blcotensor* B)
 void delete_blcotensor_device(blcotensor* D)
 {
     if (!D) return;
+    if (D->blocks)
+        for (IType b = 0; b < D->block_count; ++b) delete_block_device(D->blocks[b]);
+    delete[] D->blocks;
     blco_dev_free(D->dev);
     delete[] D->modes;
+    delete[] D->mode_masks;
+    delete[] D->mode_pos;
     delete D;
 }
 
@@ -368,6 +401,18 @@ void mttkrp_alto_dev_onemode_any(blcotensor* at_dev, IType rank, FType* output, 
     die_if(blco_mttkrp(at_dev->dev, target_mode, (int)rank, BLCO_F64, (void* const*)mats_dev, output,
                        kernel, nprtn, (int)thread_cf, nullptr),
            "mttkrp_alto_dev_onemode");
+}
+
+void mttkrp_alto_dev_onemode_ref(blcotensor* /*at_host*/, blcotensor* at_dev, KruskalModel* M, FType* output,
+                                 int kernel, int target_mode, IType thread_cf, bool /*stream_data*/, int nprtn,
+                                 FType** mats_staging_ptr, FType** /*mats_dev*/, FType** /*partial_matrices*/,
+                                 FType** /*partial_matrices_dev*/)
+{
+    if (!at_dev || !at_dev->dev || !M || !mats_staging_ptr) {
+        fprintf(stderr, "mttkrp_alto_dev_onemode: null device tensor, model or factors\n");
+        exit(EXIT_FAILURE);
+    }
+    mttkrp_alto_dev_onemode_any(at_dev, M->rank, output, kernel, target_mode, thread_cf, nprtn, mats_staging_ptr);
 }
 
 void mttkrp_alto_dev_any(blcotensor* at_host, KruskalModel* M, int kernel, int iters, int target_mode,
@@ -444,8 +489,18 @@ FType cpals_blco_dev(blcotensor* X, KruskalModel* A, int maxiter, FType toleranc
     return fit;
 }
 
-FType kruskal_fit_gpu(hipStream_t stream, FType** grams, KruskalModel* A_dev, IType nmodes,
-                      FType* last_mttkrp, IType last_rows, FType X_norm, IType mode)
+FType kruskal_fit_gpu(hipStream_t v_stream, FType** grams, KruskalModel* A, blcotensor* X, FType* /*work*/,
+                      FType* last_mttkrp, FType X_norm, IType mode, IType /*blocks*/)
+{
+    if (!A || !X || mode >= (IType)A->mode) {
+        fprintf(stderr, "kruskal_fit_gpu: bad model, tensor or mode\n");
+        exit(EXIT_FAILURE);
+    }
+    return kruskal_fit_gpu_rows(v_stream, grams, A, (IType)A->mode, last_mttkrp, X->modes[mode], X_norm, mode);
+}
+
+FType kruskal_fit_gpu_rows(hipStream_t stream, FType** grams, KruskalModel* A_dev, IType nmodes,
+                           FType* last_mttkrp, IType last_rows, FType X_norm, IType mode)
 {
     double fit = 0.0;
     die_if(blco_fit(grams, (int)nmodes, (int)A_dev->rank, A_dev->lambda, A_dev->U[mode], last_mttkrp,
@@ -466,17 +521,28 @@ void value_fill(hipStream_t stream, FType* x, IType n, FType val)
 
 void vecsum(hipStream_t stream, FType* x, IType n) { die_if(blco_vecsum(x, n, stream), "vecsum"); }
 
-int pseudoinverse_gpu(hipStream_t stream, FType** grams, IType nmodes, IType mode, IType rank,
-                      FType* M, FType* U, IType rows)
+void pseudoinverse_gpu(hipStream_t stream, FType* A, IType n, FType* /*work*/, IType /*lwork*/, int* info)
+{
+    die_if(blco_pinv(A, n, info, stream), "pseudoinverse_gpu");
+}
+
+int normal_solve_gpu(hipStream_t stream, FType** grams, IType nmodes, IType mode, IType rank, FType* M,
+                     FType* U, IType rows)
 {
     int fb = 0;
     die_if(blco_solve(grams, (int)nmodes, (int)mode, (int)rank, M, U, BLCO_F64, rows, &fb, stream),
-           "pseudoinverse_gpu");
+           "normal_solve_gpu");
     return fb;
 }
 
-void columnwise_normscal(hipStream_t stream, FType* mat, IType rank, IType mode_length, FType* lambda,
-                         mat_norm_type which)
+void columnwise_normscal(hipStream_t stream, IType /*blocks*/, FType* mat, IType rank, IType mode_length,
+                         FType* lambda)
+{
+    columnwise_normscal_which(stream, mat, rank, mode_length, lambda, MAT_NORM_2);
+}
+
+void columnwise_normscal_which(hipStream_t stream, FType* mat, IType rank, IType mode_length, FType* lambda,
+                               mat_norm_type which)
 {
     die_if(blco_colnorm(mat, BLCO_F64, mode_length, (int)rank, which == MAT_NORM_2 ? 0 : 1, lambda, stream),
            "columnwise_normscal");

@@ -340,6 +340,92 @@ __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mo
     if (tid == 0) *status = 1;
 }
 
+// --------------------------------------------------------- pseudo-inverse
+
+// In-place Moore-Penrose pseudo-inverse of a general n x n row-major matrix
+// (pseudoinverse_gpu, src/cpd_gpu.cu:452-532, which takes an SVD by gesvdj):
+// one-sided Jacobi SVD (Hestenes) in LDS.  W = A is rotated column pair by
+// column pair, the rotations accumulated in V, until every pair of columns
+// of W is orthogonal to working precision; then sigma_j = ||w_j||,
+// A = U diag(sigma) V^T with u_j = w_j / sigma_j, and
+// A^+ = V diag(1 / sigma^2) W^T over the sigma_j above the reference's
+// cutoff n * (nextafter(sigma_max, +inf) - sigma_max).  A sweep visits all
+// pairs as n' - 1 rounds of n' / 2 disjoint pairs (round-robin tournament,
+// n' = n rounded up to even), one thread per pair.  *info = 0 when a sweep
+// made no rotation, else the sweep count (gesvdj's info > 0: not converged).
+__global__ __launch_bounds__(WG) void k_pinv_general(double* __restrict__ A, int n,
+                                                     int* __restrict__ info)
+{
+    extern __shared__ double s_w[];  // W (n x n), then V (n x n), then sigma (n)
+    double* s_v = s_w + n * n;
+    double* s_sig = s_v + n * n;
+    __shared__ int s_rot;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * n; e += WG) {
+        s_w[e] = A[e];
+        s_v[e] = (e / n == e % n) ? 1.0 : 0.0;
+    }
+    const int np = (n + 1) & ~1, m = np - 1;
+    int sweeps = 0;
+    bool converged = false;
+    for (; sweeps < 60 && !converged; ++sweeps) {
+        if (tid == 0) s_rot = 0;
+        __syncthreads();
+        for (int r = 0; r < m; ++r) {
+            if (tid < np / 2) {
+                int p = tid == 0 ? r : (r + tid) % m;
+                int q = tid == 0 ? m : (r - tid + m) % m;
+                if (p > q) { const int t = p; p = q; q = t; }
+                if (q < n) {
+                    double a = 0.0, b = 0.0, g = 0.0;
+                    for (int i = 0; i < n; ++i) {
+                        const double x = s_w[i * n + p], y = s_w[i * n + q];
+                        a += x * x;
+                        b += y * y;
+                        g += x * y;
+                    }
+                    if (g != 0.0 && fabs(g) > 1e-15 * sqrt(a * b)) {
+                        const double z = (b - a) / (2.0 * g);
+                        const double t = (z >= 0 ? 1.0 : -1.0) / (fabs(z) + sqrt(1.0 + z * z));
+                        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                        for (int i = 0; i < n; ++i) {
+                            const double x = s_w[i * n + p], y = s_w[i * n + q];
+                            s_w[i * n + p] = c * x - s * y;
+                            s_w[i * n + q] = s * x + c * y;
+                            const double vx = s_v[i * n + p], vy = s_v[i * n + q];
+                            s_v[i * n + p] = c * vx - s * vy;
+                            s_v[i * n + q] = s * vx + c * vy;
+                        }
+                        s_rot = 1;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        converged = s_rot == 0;  // every thread reads it after the round's barrier
+        __syncthreads();
+    }
+    for (int j = tid; j < n; j += WG) {
+        double a = 0.0;
+        for (int i = 0; i < n; ++i) a += s_w[i * n + j] * s_w[i * n + j];
+        s_sig[j] = sqrt(a);
+    }
+    __syncthreads();
+    double smax = 0.0;
+    for (int j = 0; j < n; ++j) smax = fmax(smax, s_sig[j]);
+    const double tol = n * (nextafter(smax, INFINITY) - smax);
+    for (int e = tid; e < n * n; e += WG) {
+        const int i = e / n, k = e % n;
+        double t = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double sg = s_sig[j];
+            if (sg > tol) t += s_v[i * n + j] * s_w[k * n + j] / (sg * sg);
+        }
+        A[e] = t;
+    }
+    if (tid == 0 && info) *info = converged ? 0 : sweeps;
+}
+
 // ------------------------------------------------------------ solve rows
 
 // U = M F with F = V^{-1} (or V^+), row by row, written as the factor dtype,
@@ -626,6 +712,15 @@ uint32_t solve_parts(uint64_t rows)
 {
     const uint64_t p = (rows + 31) / 32;
     return (uint32_t)(p < 1 ? 1 : p > 2048 ? 2048 : p);
+}
+
+int launch_pinv_general(double* A, int n, int* info, hipStream_t st)
+{
+    if (n < 1 || n > 64) return fail(BLCO_EUNSUPPORTED, "pseudo-inverse of order %d outside [1, 64]", n);
+    const size_t lds = sizeof(double) * ((size_t)2 * n * n + n);
+    hipLaunchKernelGGL(k_pinv_general, dim3(1), dim3(WG), lds, st, A, n, info);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
 }
 
 int launch_lambda(const double* stats, int nparts, int R, int which, double* lambda, double* inv,

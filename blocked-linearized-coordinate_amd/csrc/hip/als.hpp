@@ -27,6 +27,9 @@ double gram_eps(int factor_dtype);
 // scales the Cholesky pivot floor and the pseudo-inverse tolerance.
 int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps, double* fac,
                      double* work, int* status, hipStream_t st);
+// In-place pseudo-inverse of a general n x n matrix (n <= 64); info
+// (device, may be null): 0 converged, else the sweep count.
+int launch_pinv_general(double* A, int n, int* info, hipStream_t st);
 uint32_t solve_parts(uint64_t rows);
 // stats: solve_parts(rows) * R doubles.
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,

@@ -985,107 +985,75 @@ int blco_device_count(void)
     return n;
 }
 
-int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos,
-                           uint64_t nnz, const uint64_t* coords, const double* vals,
-                           uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz,
-                           const uint64_t* block_coords, int value_dtype, int stream_data,
-                           uint64_t slot_nnz, blco_dev** out)
+}  // extern "C"
+
+namespace blco {
+
+// Device tensor from host arrays (uploaded), or from device arrays that it
+// takes over (adopt_coords / adopt_vals: resident only; `coords` is then a
+// host copy of the device words for the host-side tables and `vals` is
+// unused).  Adopted arrays belong to the tensor (freed by blco_dev_free),
+// and are freed here on failure.
+int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos,
+                            uint64_t nnz, const uint64_t* coords, const double* vals,
+                            uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz,
+                            const uint64_t* block_coords, int value_dtype, int stream_data,
+                            uint64_t slot_nnz, uint64_t* adopt_coords, void* adopt_vals, blco_dev** out)
 {
-    if (!out) return fail(BLCO_EINVAL, "null output");
+    const bool adopt = adopt_coords != nullptr;
+    auto drop_adopted = [&](int rc) {
+        if (adopt_coords) (void)hipFree(adopt_coords);
+        if (adopt_vals) (void)hipFree(adopt_vals);
+        return rc;
+    };
+    if (!out) return drop_adopted(fail(BLCO_EINVAL, "null output"));
     *out = nullptr;
-    if (nmodes < 1 || nmodes > BLCO_MAX_MODES) return fail(BLCO_EINVAL, "nmodes %d", nmodes);
-    if (!dims || !masks || !pos) return fail(BLCO_EINVAL, "null layout");
-    if (nnz && (!coords || !vals || !block_start || !block_nnz || !block_coords || nblocks == 0))
-        return fail(BLCO_EINVAL, "null tensor arrays");
+    if (nmodes < 1 || nmodes > BLCO_MAX_MODES) return drop_adopted(fail(BLCO_EINVAL, "nmodes %d", nmodes));
+    if (!dims || !masks || !pos) return drop_adopted(fail(BLCO_EINVAL, "null layout"));
+    if (adopt && (stream_data || !adopt_vals))
+        return drop_adopted(fail(BLCO_EINVAL, "adopted device arrays must be resident"));
+    if (nnz && (!coords || (!vals && !adopt) || !block_start || !block_nnz || !block_coords || nblocks == 0))
+        return drop_adopted(fail(BLCO_EINVAL, "null tensor arrays"));
     if (value_dtype != BLCO_F64 && value_dtype != BLCO_F32)
-        return fail(BLCO_EINVAL, "value dtype %d", value_dtype);
+        return drop_adopted(fail(BLCO_EINVAL, "value dtype %d", value_dtype));
     uint64_t sum = 0;
     for (uint64_t b = 0; b < nblocks; ++b) {
-        if (block_start[b] != sum) return fail(BLCO_EINVAL, "blocks are not contiguous at %llu", (unsigned long long)b);
+        if (block_start[b] != sum) return drop_adopted(fail(BLCO_EINVAL, "blocks are not contiguous at %llu", (unsigned long long)b));
         sum += block_nnz[b];
     }
-    if (sum != nnz) return fail(BLCO_EINVAL, "block sizes sum to %llu, nnz is %llu", (unsigned long long)sum, (unsigned long long)nnz);
+    if (sum != nnz) return drop_adopted(fail(BLCO_EINVAL, "block sizes sum to %llu, nnz is %llu", (unsigned long long)sum, (unsigned long long)nnz));
     for (int m = 0; m < nmodes; ++m) {
-        if (dims[m] == 0) return fail(BLCO_EINVAL, "mode %d has length 0", m);
-        if (masks[m] && (pos[m] < 0 || pos[m] > 63)) return fail(BLCO_EINVAL, "mode %d position %d", m, pos[m]);
+        if (dims[m] == 0) return drop_adopted(fail(BLCO_EINVAL, "mode %d has length 0", m));
+        if (masks[m] && (pos[m] < 0 || pos[m] > 63)) return drop_adopted(fail(BLCO_EINVAL, "mode %d position %d", m, pos[m]));
     }
-    // every decoded index must lie inside its mode: kernels gather factor
-    // rows and scatter output rows with these indices unchecked.  The same
-    // pass records each fine chunk's per-mode row range.
-    std::vector<uint64_t> fbeg, flo, fhi;
-    std::vector<uint32_t> flen, fblk;
-    for (uint64_t b = 0; b < nblocks; ++b)
-        for (uint64_t s0 = 0; s0 < block_nnz[b]; s0 += FINE_CHUNK) {
-            fbeg.push_back(block_start[b] + s0);
-            flen.push_back((uint32_t)std::min<uint64_t>(FINE_CHUNK, block_nnz[b] - s0));
-            fblk.push_back((uint32_t)b);
-        }
-    const uint64_t nfine = fbeg.size();
+    // every decoded index must lie inside its mode (kernels gather factor
+    // rows and scatter output rows with these indices unchecked); the same
+    // passes find the hot rows and each fine chunk's per-mode row range
     progress("device tensor: checking %llu nonzeros in %llu blocks", (unsigned long long)nnz,
              (unsigned long long)nblocks);
-    // pass 1: bounds check and per-mode row counts
-    std::vector<std::vector<uint32_t>> hot(nmodes);
-    std::vector<std::vector<uint64_t>> hot_cnt(nmodes);
-    for (int m = 0; m < nmodes; ++m) {
-        std::vector<uint32_t> cnt;
-        const bool count = dims[m] <= (1ull << 31);
-        if (count) cnt.assign(dims[m], 0);
-        for (uint64_t b = 0; b < nblocks; ++b) {
-            const uint64_t bc = block_coords[b * nmodes + m], mk = masks[m];
-            const int ps = mk ? pos[m] : 0;
-            for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
-                const uint64_t v = ((coords[i] >> ps) & mk) | bc;
-                if (v >= dims[m])
-                    return fail(BLCO_EINVAL, "block %llu decodes mode-%d index %llu >= %llu",
-                                (unsigned long long)b, m, (unsigned long long)v,
-                                (unsigned long long)dims[m]);
-                if (count && cnt[v] != 0xFFFFFFFFu) ++cnt[v];
-            }
-        }
-        if (count && nnz) {
-            const uint64_t thr = std::max<uint64_t>(2, nnz / 2048);
-            std::vector<uint32_t> cand;
-            for (uint64_t r = 0; r < dims[m]; ++r)
-                if (cnt[r] >= thr) cand.push_back((uint32_t)r);
-            std::sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) {
-                return cnt[x] != cnt[y] ? cnt[x] > cnt[y] : x < y;
-            });
-            if (cand.size() > HOT_EXCLUDE) cand.resize(HOT_EXCLUDE);
-            hot[m] = cand;
-            for (uint32_t r : cand) hot_cnt[m].push_back(cnt[r]);
-        }
+    TensorTables tt;
+    {
+        int rc = analyze_tensor(nmodes, dims, masks, pos, nnz, coords, nblocks, block_start, block_nnz,
+                                block_coords, FINE_CHUNK, HOT_EXCLUDE, &tt);
+        if (rc) return drop_adopted(rc);
     }
-    // pass 2: per fine chunk, per mode, the row range of the non-hot rows
-    flo.assign((uint64_t)nmodes * nfine, ~0ull);
-    fhi.assign((uint64_t)nmodes * nfine, 0);
-    for (int m = 0; m < nmodes; ++m) {
-        std::vector<uint8_t> is_hot;
-        if (!hot[m].empty()) {
-            is_hot.assign(dims[m], 0);
-            for (uint32_t r : hot[m]) is_hot[r] = 1;
-        }
-        const uint64_t mk = masks[m];
-        const int ps = mk ? pos[m] : 0;
-        for (uint64_t f = 0; f < nfine; ++f) {
-            const uint64_t bc = block_coords[(uint64_t)fblk[f] * nmodes + m];
-            uint64_t mn = ~0ull, mx = 0;
-            for (uint64_t i = fbeg[f]; i < fbeg[f] + flen[f]; ++i) {
-                const uint64_t v = ((coords[i] >> ps) & mk) | bc;
-                if (!is_hot.empty() && is_hot[v]) continue;
-                mn = v < mn ? v : mn;
-                mx = v > mx ? v : mx;
-            }
-            flo[m * nfine + f] = mn;
-            fhi[m * nfine + f] = mx;
-        }
-    }
-    if (!have_kfd()) return fail(BLCO_ENODEV, "no GPU (/dev/kfd absent)");
+    std::vector<uint64_t>& fbeg = tt.fbeg;
+    std::vector<uint32_t>& flen = tt.flen;
+    std::vector<uint32_t>& fblk = tt.fblk;
+    std::vector<uint64_t>& flo = tt.flo;
+    std::vector<uint64_t>& fhi = tt.fhi;
+    std::vector<std::vector<uint32_t>>& hot = tt.hot;
+    std::vector<std::vector<uint64_t>>& hot_cnt = tt.hot_cnt;
+    if (!have_kfd()) return drop_adopted(fail(BLCO_ENODEV, "no GPU (/dev/kfd absent)"));
     if (stream_data && slot_nnz == 0) slot_nnz = 1ull << 24;  // --max-block-size default
     // a slot holds at least one chunk at the largest tiles-per-wave (64 x 64)
     if (stream_data && slot_nnz < 4096) slot_nnz = 4096;
 
     blco_dev* d = new (std::nothrow) blco_dev();
-    if (!d) return fail(BLCO_ENOMEM, "out of host memory");
+    if (!d) return drop_adopted(fail(BLCO_ENOMEM, "out of host memory"));
+    // adopted arrays belong to the tensor from here on (blco_dev_free)
+    d->d_coords = adopt_coords;
+    d->d_vals = adopt_vals;
     d->nmodes = nmodes;
     for (int m = 0; m < nmodes; ++m) {
         d->dims[m] = dims[m];
@@ -1145,7 +1113,9 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
             for (uint64_t i = 0; i < cnt; ++i) f[i] = (float)vals[off + i];
         }
     };
-    if (!d->stream_data) {
+    if (adopt) {
+        d->bytes += nnz * (8 + vsz);
+    } else if (!d->stream_data) {
         if (nnz) {
             if (hipMalloc(&d->d_coords, nnz * 8) != hipSuccess || hipMalloc(&d->d_vals, nnz * vsz) != hipSuccess)
                 return cleanup(fail(BLCO_ENOMEM, "device allocation of %llu nonzeros failed",
@@ -1187,6 +1157,20 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
     // every block must fit a slot when streaming (chunks never straddle blocks)
     *out = d;
     return BLCO_OK;
+}
+
+}  // namespace blco
+
+extern "C" {
+
+int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos,
+                           uint64_t nnz, const uint64_t* coords, const double* vals,
+                           uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz,
+                           const uint64_t* block_coords, int value_dtype, int stream_data,
+                           uint64_t slot_nnz, blco_dev** out)
+{
+    return create_device_tensor(nmodes, dims, masks, pos, nnz, coords, vals, nblocks, block_start, block_nnz,
+                                block_coords, value_dtype, stream_data, slot_nnz, nullptr, nullptr, out);
 }
 
 int blco_dev_create(const blco_host* h, int value_dtype, int stream_data, uint64_t slot_nnz,
@@ -1245,6 +1229,17 @@ void blco_dev_free(blco_dev* d)
 }
 
 uint64_t blco_dev_bytes(const blco_dev* d) { return d ? d->bytes : 0; }
+
+int blco_dev_arrays(const blco_dev* d, uint64_t** coords, void** vals, int* value_dtype, uint64_t** block_coords)
+{
+    if (!d) return fail(BLCO_EINVAL, "null device tensor");
+    const bool res = !d->stream_data;
+    if (coords) *coords = res ? d->d_coords : nullptr;
+    if (vals) *vals = res ? d->d_vals : nullptr;
+    if (value_dtype) *value_dtype = d->vdtype;
+    if (block_coords) *block_coords = res ? d->d_bcoords : nullptr;
+    return BLCO_OK;
+}
 
 int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype, void* const* factors, void* out,
                 int kernel, int nprtn, int thread_cf, void* stream)
@@ -1360,6 +1355,18 @@ int blco_solve(double* const* grams, int nmodes, int mode, int rank, const void*
     if (rc) return rc;
     HIP_TRY(e);
     if (used_fallback) *used_fallback = h_status;
+    return BLCO_OK;
+}
+
+int blco_pinv(double* A, uint64_t n, int* info, void* stream)
+{
+    if (!A) return fail(BLCO_EINVAL, "null matrix");
+    if (n < 1 || n > 64) return fail(BLCO_EUNSUPPORTED, "pseudo-inverse of order %llu outside [1, 64]",
+                                     (unsigned long long)n);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = launch_pinv_general(A, (int)n, info, st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
     return BLCO_OK;
 }
 

@@ -103,17 +103,13 @@ static inline uint64_t pext64(uint64_t x, uint64_t m) { return _pext_u64(x, m); 
 // key half (hi[i] for sorted nonzero i; none: one run), each split into
 // pieces of at most mbs nonzeros; block coordinates are the upper half's
 // bits of every mode, shifted above its truncated lower bits.
-void build_blocks(const Layout& L, uint64_t nnz, const uint64_t* hi, uint64_t mbs, blco_host* h)
+void build_blocks_runs(const Layout& L, uint64_t nnz, uint64_t nruns, const uint64_t* start, const uint64_t* hival,
+                       uint64_t mbs, blco_host* h)
 {
     const int N = L.nmodes;
-    uint64_t i = 0;
-    while (i < nnz) {
-        const uint64_t bid = hi ? hi[i] : 0;
-        uint64_t j = i + 1;
-        if (hi)
-            while (j < nnz && hi[j] == bid) ++j;
-        else
-            j = nnz;
+    for (uint64_t r = 0; r < nruns; ++r) {
+        const uint64_t i = start[r], j = r + 1 < nruns ? start[r + 1] : nnz;
+        const uint64_t bid = hival ? hival[r] : 0;
         for (uint64_t s = i; s < j; s += mbs) {
             const uint64_t cnt = (j - s) < mbs ? (j - s) : mbs;
             h->block_start.push_back(s);
@@ -122,8 +118,19 @@ void build_blocks(const Layout& L, uint64_t nnz, const uint64_t* hi, uint64_t mb
             for (int n = 0; n < N; ++n)
                 h->block_coords.push_back(L.alto_hi[n] ? pext64(bid, L.alto_hi[n]) << L.trunc[n] : 0);
         }
-        i = j;
     }
+}
+
+void build_blocks(const Layout& L, uint64_t nnz, const uint64_t* hi, uint64_t mbs, blco_host* h)
+{
+    // runs of equal upper key half (one run without upper halves)
+    std::vector<uint64_t> start, val;
+    for (uint64_t i = 0; i < nnz; ++i)
+        if (i == 0 || (hi && hi[i] != hi[i - 1])) {
+            start.push_back(i);
+            val.push_back(hi ? hi[i] : 0);
+        }
+    build_blocks_runs(L, nnz, start.size(), start.data(), val.data(), mbs, h);
 }
 
 template <typename K, typename P>

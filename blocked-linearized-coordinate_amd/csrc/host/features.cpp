@@ -93,11 +93,21 @@ void compute_static_features(int N, const uint64_t* dims, const uint64_t* masks,
         std::vector<uint64_t> sizes;
         if (dims[n] <= (1ull << 28)) {
             std::vector<uint32_t> cnt(dims[n], 0);
-            for (uint64_t b = 0; b < nblocks; ++b)
-                for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
-                    uint32_t& c = cnt[index(b, i, n)];
-                    c += c != 0xFFFFFFFFu;
+            uint32_t* cp = cnt.data();
+            if (nnz < 0xFFFFFFFFull) {
+                // no count can reach the saturation value: atomic increments in parallel
+                for (uint64_t b = 0; b < nblocks; ++b) {
+                    #pragma omp parallel for schedule(static)
+                    for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i)
+                        __atomic_fetch_add(&cp[index(b, i, n)], 1u, __ATOMIC_RELAXED);
                 }
+            } else {
+                for (uint64_t b = 0; b < nblocks; ++b)
+                    for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
+                        uint32_t& c = cp[index(b, i, n)];
+                        c += c != 0xFFFFFFFFu;
+                    }
+            }
             for (uint32_t c : cnt)
                 if (c) sizes.push_back(c);
         } else {

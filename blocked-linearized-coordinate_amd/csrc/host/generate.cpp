@@ -24,8 +24,36 @@ typedef unsigned __int128 u128;
 
 namespace blco {
 
+void sampler_tables(uint64_t I, double s, std::vector<double>* cdf, std::vector<uint64_t>* guide)
+{
+    cdf->clear();
+    guide->clear();
+    if (!(s > 0.0)) return;
+    cdf->resize(I);
+    double* c = cdf->data();
+    double acc = 0.0;
+    for (uint64_t k = 0; k < I; ++k) {
+        acc += std::pow((double)(k + 1), -s);
+        c[k] = acc;
+    }
+    for (uint64_t k = 0; k < I; ++k) c[k] /= acc;
+    c[I - 1] = 1.0;
+    guide->resize(SAMPLER_GUIDE + 1);
+    for (uint32_t g = 0; g <= SAMPLER_GUIDE; ++g)
+        (*guide)[g] = (uint64_t)(std::upper_bound(c, c + I, (double)g / SAMPLER_GUIDE) - c);
+}
+
+void generating_factor(uint64_t seed, int mode, uint64_t rows, int Rg, double* a)
+{
+    const uint64_t cnt = rows * (uint64_t)Rg;
+    #pragma omp parallel for schedule(static)
+    for (uint64_t e = 0; e < cnt; ++e) a[e] = -std::log1p(-rng_u01(seed, STREAM_GEN_FACTOR + (uint64_t)mode, e));
+}
+
+double log1p_count(uint64_t c) { return std::log1p((double)c); }
+
 struct IndexSampler {
-    static constexpr uint32_t GUIDE = 1u << 16;
+    static constexpr uint32_t GUIDE = SAMPLER_GUIDE;
     uint64_t I = 1;
     double s = 0.0;
     std::vector<double> cdf;      // Zipf only
@@ -35,19 +63,7 @@ struct IndexSampler {
     {
         I = len;
         s = zs;
-        if (s > 0.0) {
-            cdf.resize(I);
-            double acc = 0.0;
-            for (uint64_t k = 0; k < I; ++k) {
-                acc += std::pow((double)(k + 1), -s);
-                cdf[k] = acc;
-            }
-            for (uint64_t k = 0; k < I; ++k) cdf[k] /= acc;
-            cdf[I - 1] = 1.0;
-            guide.resize(GUIDE + 1);
-            for (uint32_t g = 0; g <= GUIDE; ++g)
-                guide[g] = (uint64_t)(std::upper_bound(cdf.begin(), cdf.end(), (double)g / GUIDE) - cdf.begin());
-        }
+        sampler_tables(I, s, &cdf, &guide);
     }
     // Inverse CDF: the first k with cdf[k] > u (an upper bound search over
     // the guide interval [guide[g], guide[g + 1]] that holds it).
@@ -203,11 +219,7 @@ extern "C" int blco_generate(const blco_gen_spec* sp, blco_coo** out)
     std::vector<std::vector<double>> A(N);
     for (int n = 0; n < N; ++n) {
         A[n].resize(sp->dims[n] * (uint64_t)Rg);
-        double* a = A[n].data();
-        uint64_t cnt = sp->dims[n] * (uint64_t)Rg;
-        #pragma omp parallel for schedule(static)
-        for (uint64_t e = 0; e < cnt; ++e)
-            a[e] = -std::log1p(-rng_u01(sp->seed, STREAM_GEN_FACTOR + n, e));
+        generating_factor(sp->seed, n, sp->dims[n], Rg, A[n].data());
     }
     progress("generate: generating factors drawn");
     #pragma omp parallel for schedule(static)

@@ -62,6 +62,34 @@ void row_order(const RowOrderSpec& o, uint64_t nnz, const uint64_t* coords, uint
 // dst[i] = src[perm[i]] for esz-byte elements.
 void gather_bytes(const void* src, size_t esz, const uint64_t* perm, uint64_t n, void* dst);
 
+// Pieces of the synthetic-tensor generator (generate.cpp) that the GPU
+// generator (gen_dev.hip) takes from the host so that both form the same
+// doubles: the Zipf sampler's tables of a mode of length I (cdf[k] =
+// sum_{j<=k} (j+1)^-s / sum_j (j+1)^-s, and guide[g] = the first k with
+// cdf[k] > g / SAMPLER_GUIDE; both empty for s = 0, a uniform mode), the
+// Exp(1) generating factor of a mode (rows x Rg), and log1p of a count.
+constexpr uint32_t SAMPLER_GUIDE = 1u << 16;
+void sampler_tables(uint64_t I, double s, std::vector<double>* cdf, std::vector<uint64_t>* guide);
+void generating_factor(uint64_t seed, int mode, uint64_t rows, int Rg, double* out);
+double log1p_count(uint64_t c);
+
+// Host-side tables of a device tensor (analyze.cpp): fine chunks of at most
+// fine_chunk nonzeros within each block (first nonzero, length, block), the
+// hot rows of each mode (count >= max(2, nnz / 2048), at most hot_max, by
+// decreasing count, and their counts), and per mode and fine chunk the row
+// range [flo, fhi] of the non-hot rows (flo = ~0 when none).  Fails when an
+// index decodes outside its mode.
+struct TensorTables {
+    std::vector<uint64_t> fbeg, flo, fhi;
+    std::vector<uint32_t> flen, fblk;
+    std::vector<std::vector<uint32_t>> hot;
+    std::vector<std::vector<uint64_t>> hot_cnt;
+};
+int analyze_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos, uint64_t nnz,
+                   const uint64_t* coords, uint64_t nblocks, const uint64_t* block_start,
+                   const uint64_t* block_nnz, const uint64_t* block_coords, uint32_t fine_chunk, uint32_t hot_max,
+                   TensorTables* t);
+
 // Kernel-selector features of every mode that depend on the tensor only
 // (entries 0-3, 5, 7-22 of each row of `stride` floats; features.cpp).
 void compute_static_features(int N, const uint64_t* dims, const uint64_t* masks, const int* pos, uint64_t nnz,
@@ -95,4 +123,8 @@ namespace blco {
 // sorted nonzero i, or nullptr for keys of at most 64 bits): runs of equal
 // upper half split into pieces of at most mbs nonzeros (encode.cpp).
 void build_blocks(const Layout& L, uint64_t nnz, const uint64_t* hi, uint64_t mbs, blco_host* h);
+// The same from the runs themselves: run r starts at start[r] (ascending,
+// start[0] = 0) and has upper half hival[r] (hival may be nullptr: 0).
+void build_blocks_runs(const Layout& L, uint64_t nnz, uint64_t nruns, const uint64_t* start, const uint64_t* hival,
+                       uint64_t mbs, blco_host* h);
 }  // namespace blco

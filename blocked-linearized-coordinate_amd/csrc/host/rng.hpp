@@ -41,7 +41,9 @@ enum : uint64_t {
 // (seed, STREAM_VALUE, base + j), j = 0, 1, ...  lam <= 30: inversion of the
 // truncated CDF; lam > 30: PTRS transformed rejection (Hoermann 1993,
 // "The transformed rejection method for generating Poisson random
-// variables"), redrawn on 0.
+// variables"), redrawn on 0.  Every multiply-add that is fused is written
+// as std::fma and the file is compiled without contraction, so the GPU
+// generator (the same steps in gen_dev.hip) forms the same doubles.
 inline uint64_t ztpoisson(double lam, uint64_t seed, uint64_t base)
 {
     uint64_t j = 0;
@@ -49,7 +51,7 @@ inline uint64_t ztpoisson(double lam, uint64_t seed, uint64_t base)
     if (lam <= 30.0) {
         double p0 = std::exp(-lam);
         double u = rng_u01(seed, STREAM_VALUE, base + j++);
-        double target = p0 + u * (1.0 - p0);  // P(K <= k) crossing, K >= 1
+        double target = std::fma(u, 1.0 - p0, p0);  // P(K <= k) crossing, K >= 1
         double p = p0, c = p0;
         uint64_t k = 0;
         while (c < target && k < 10000) {
@@ -60,22 +62,22 @@ inline uint64_t ztpoisson(double lam, uint64_t seed, uint64_t base)
         return k < 1 ? 1 : k;
     }
     double slam = std::sqrt(lam), loglam = std::log(lam);
-    double b = 0.931 + 2.53 * slam;
-    double a = -0.059 + 0.02483 * b;
+    double b = std::fma(slam, 2.53, 0.931);
+    double a = std::fma(b, 0.02483, -0.059);
     double invalpha = 1.1239 + 1.1328 / (b - 3.4);
     double vr = 0.9277 - 3.6224 / (b - 2.0);
     for (;;) {
         double U = rng_u01(seed, STREAM_VALUE, base + j++) - 0.5;
         double V = rng_u01(seed, STREAM_VALUE, base + j++);
         double us = 0.5 - std::fabs(U);
-        double kf = std::floor((2.0 * a / us + b) * U + lam + 0.43);
+        double kf = std::floor(std::fma(2.0 * a / us + b, U, lam) + 0.43);
         if (us >= 0.07 && V <= vr) {
             if (kf >= 1.0) return (uint64_t)kf;
             continue;
         }
         if (kf < 0.0 || (us < 0.013 && V > us)) continue;
         if (std::log(V) + std::log(invalpha) - std::log(a / (us * us) + b) <=
-            -lam + kf * loglam - std::lgamma(kf + 1.0)) {
+            std::fma(kf, loglam, -lam) - std::lgamma(kf + 1.0)) {
             if (kf >= 1.0) return (uint64_t)kf;
         }
     }

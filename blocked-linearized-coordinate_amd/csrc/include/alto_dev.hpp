@@ -28,7 +28,32 @@ inline void mttkrp_alto_dev(blcotensor* at_host, KruskalModel* M, int kernel, in
                         do_batching, nprtn);
 }
 
-// One MTTKRP into device `output` (dims[target_mode] x rank f64, accumulated)
-// from device factors mats_dev (f64) (src/alto_dev.cu:1954-2032).
+// mttkrp_alto_dev_onemode (include/alto_dev.hpp:26-27,
+// src/alto_dev.cu:1954-2032): output (device, modes[target_mode] x rank f64)
+// += the mode-target_mode MTTKRP of at_dev with the device factors
+// mats_staging_ptr[m] (a host array of device pointers, as the reference's
+// M_dev->U).  The rank is M->rank; stream_data follows the device tensor
+// (it was fixed when at_dev was made).  mats_dev and the partial matrices
+// are accepted for the reference's call sequence: the library's kernels
+// take the factor pointers by value and keep their partial matrices with
+// the device tensor.
+void mttkrp_alto_dev_onemode_ref(blcotensor* at_host, blcotensor* at_dev, KruskalModel* M,
+                                 FType* output, int kernel, int target_mode, IType thread_cf,
+                                 bool stream_data, int nprtn, FType** mats_staging_ptr,
+                                 FType** mats_dev, FType** partial_matrices,
+                                 FType** partial_matrices_dev);
+template <typename LIT>
+inline void mttkrp_alto_dev_onemode(blcotensor* at_host, blcotensor* at_dev, KruskalModel* M,
+                                    FType* output, int kernel, int target_mode, IType thread_cf,
+                                    bool stream_data, int nprtn, FType** mats_staging_ptr,
+                                    FType** mats_dev, FType** partial_matrices,
+                                    FType** partial_matrices_dev)
+{
+    mttkrp_alto_dev_onemode_ref(at_host, at_dev, M, output, kernel, target_mode, thread_cf,
+                                stream_data, nprtn, mats_staging_ptr, mats_dev, partial_matrices,
+                                partial_matrices_dev);
+}
+
+// The same from explicit arguments: rank and device factors (f64).
 void mttkrp_alto_dev_onemode_any(blcotensor* at_dev, IType rank, FType* output, int kernel,
                                  int target_mode, IType thread_cf, int nprtn, FType** mats_dev);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include "alto.hpp"
 #include "common.hpp"
 #include "sptensor.hpp"
 
@@ -11,14 +12,18 @@ struct blcotensor;
 struct blco_host;
 struct blco_dev;
 
-// One block: nonzeros sharing the upper key bits (blco.hpp:11-20).  Host
-// blocks point into the parent's arrays; device blocks into device memory.
-struct blco_block {
-    int nmode = 0;
-    IType nnz = 0;
-    IType* idx = nullptr;           // BLCO coordinates of the block
-    FType* vals = nullptr;          // values (host blocks)
-    IType* block_coords = nullptr;  // nmode upper-bit coordinates
+// One block: nonzeros sharing the upper key bits (blco.hpp:11-20).  As in
+// the reference it extends the ALTO record: nmode, nnz, idx, vals, and the
+// parent's dims, mode_masks and mode_pos.  Host blocks point into the
+// parent's arrays; the blocks of a device tensor point at its device arrays
+// when the tensor is resident (idx and vals are null for a streamed tensor,
+// whose blocks pass through device slots).
+struct blco_block : public AltoTensor<IType> {
+    IType* block_coords = nullptr;  // nmode upper-bit coordinates (host)
+    // lvl3 partial matrices of the reference's kernels; this library keeps
+    // its own with the device tensor, so these stay null
+    FType** pmatrices_staging_ptr = nullptr;
+    FType** pmatrices = nullptr;
     blcotensor* parent = nullptr;
 };
 
@@ -32,9 +37,19 @@ struct blcotensor {
     IType block_count = 0;
     IType max_block_size = 0;
     blco_block** blocks = nullptr;
+    // Device copies of the block records for the reference's batched
+    // kernels (src/blco.cu:12-58) and their tile / warp tables; this
+    // library's kernels take the device tensor handle (dev) and build their
+    // own chunk tables, so these stay null.
+    blco_block** blocks_dev_staging = nullptr;
+    blco_block** blocks_dev_ptr = nullptr;
+    IType* tile_info = nullptr;
     IType* coords = nullptr;  // total_nnz BLCO coordinates (host)
     FType* values = nullptr;  // total_nnz values (host)
     hipStream_t* streams = nullptr;
+    IType* warp_info = nullptr;
+    IType* warp_info_gpu = nullptr;
+    IType warp_info_length = 0;
     // implementation handles
     blco_host* host = nullptr;  // host tensors
     blco_dev* dev = nullptr;    // device tensors

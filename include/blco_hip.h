@@ -97,6 +97,13 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims,
 void blco_dev_free(blco_dev* d);
 /* Bytes of device memory held by the tensor. */
 uint64_t blco_dev_bytes(const blco_dev* d);
+/* Device arrays of a resident tensor (the blocks of gen_blcotensor_device
+ * point into them, src/blco.cu:12-58): BLCO coordinates (u64[nnz], block
+ * after block), values (value_dtype elements) and the block coordinates
+ * (u64[nblocks * nmodes]).  All three are NULL for a streamed tensor, whose
+ * blocks stay in pinned host memory. */
+int blco_dev_arrays(const blco_dev* d, uint64_t** coords, void** vals, int* value_dtype,
+                    uint64_t** block_coords);
 
 /* ----------------------------------------------------------------- MTTKRP */
 
@@ -145,6 +152,14 @@ int blco_vecsum(double* x, uint64_t n, void* stream);
 int blco_solve(double* const* grams, int nmodes, int mode, int rank,
                const void* M, void* U, int dtype, uint64_t rows,
                int* used_fallback, void* stream);
+
+/* pseudoinverse_gpu, src/cpd_gpu.cu:452-532: A (device, n x n row-major
+ * f64, n <= 64) is overwritten with its Moore-Penrose pseudo-inverse; the
+ * singular values at or below n * (nextafter(s_max, +inf) - s_max) are
+ * dropped (src/cpd_gpu.cu:487-489).  One-sided Jacobi SVD in one
+ * workgroup.  info (device int, may be NULL): 0 when the SVD converged,
+ * else the sweep count.  Synchronises the stream. */
+int blco_pinv(double* A, uint64_t n, int* info, void* stream);
 
 /* Column normalisation of one factor (KruskalModelNorm,
  * src/kruskal_model.cpp:142-259; columnwise_normscal, src/cpd_gpu.cu:244-304
