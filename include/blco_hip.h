@@ -153,6 +153,20 @@ int blco_solve(double* const* grams, int nmodes, int mode, int rank,
                const void* M, void* U, int dtype, uint64_t rows,
                int* used_fallback, void* stream);
 
+/* blco_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as blco_solve; the solve's scratch is a
+ * file-scope device array of blco_solve_noalloc_scratch_bytes() bytes per
+ * GPU (held until the process ends), which bounds rank to 64.  Asynchronous
+ * on `stream` unless used_fallback is non-NULL (then the flag is read back
+ * and the stream synchronised).  Calls that overlap on one GPU share that
+ * array and race: serialise them. */
+int blco_solve_noalloc(double* const* grams, int nmodes, int mode, int rank,
+                       const void* M, void* U, int dtype, uint64_t rows,
+                       int* used_fallback, void* stream);
+uint64_t blco_solve_noalloc_scratch_bytes(void);
+
 /* pseudoinverse_gpu, src/cpd_gpu.cu:452-532: A (device, n x n row-major
  * f64, n <= 64) is overwritten with its Moore-Penrose pseudo-inverse; the
  * singular values at or below n * (nextafter(s_max, +inf) - s_max) are
