@@ -100,6 +100,13 @@ def lib():
                                                  C.c_uint64, u64p, u64p, u64p, C.c_int, C.c_int,
                                                  C.c_uint64, C.POINTER(vp)]),
             "blco_dev_free": (None, [vp]),
+            "blco_generate_device": (C.c_int, [C.POINTER(GenSpec), C.c_uint64, C.c_int, C.POINTER(vp)]),
+            "blco_dev_to_host": (C.c_int, [vp, C.POINTER(vp)]),
+            "blco_dev_info": (C.c_int, [vp, C.POINTER(C.c_int), u64p, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64)]),
+            "blco_dev_sample": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+            "blco_dev_set_value_dtype": (C.c_int, [vp, C.c_int]),
+            "blco_dev_row_copy": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, u64p, vp, vp, vp, vp]),
             "blco_dev_bytes": (C.c_uint64, [vp]),
             "blco_mttkrp": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), vp, C.c_int,
                                       C.c_int, C.c_int, vp]),
@@ -289,16 +296,7 @@ class Coo:
 
     @classmethod
     def generate(cls, dims, nnz, zipf=None, gen_rank=10, value_kind=0, seed=1, topup=False):
-        s = GenSpec()
-        s.nmodes = len(dims)
-        for i, d in enumerate(dims):
-            s.dims[i] = int(d)
-            s.zipf_s[i] = float(zipf[i]) if zipf is not None else 0.0
-        s.target_nnz = int(nnz)
-        s.topup = 1 if topup else 0
-        s.gen_rank = gen_rank
-        s.value_kind = value_kind
-        s.seed = seed
+        s = _gen_spec(dims, nnz, zipf, gen_rank, value_kind, seed, topup)
         h = vp()
         _check(lib().blco_generate(C.byref(s), C.byref(h)))
         return cls(h.value)
@@ -434,6 +432,20 @@ def encode_device(coo_or_dims, cidx=None, vals=None, max_block_size=0):
 
 # ------------------------------------------------------------ device side
 
+def _gen_spec(dims, nnz, zipf, gen_rank, value_kind, seed, topup):
+    s = GenSpec()
+    s.nmodes = len(dims)
+    for i, d in enumerate(dims):
+        s.dims[i] = int(d)
+        s.zipf_s[i] = float(zipf[i]) if zipf is not None else 0.0
+    s.target_nnz = int(nnz)
+    s.topup = 1 if topup else 0
+    s.gen_rank = gen_rank
+    s.value_kind = value_kind
+    s.seed = seed
+    return s
+
+
 class DeviceTensor:
     """BLCO tensor in HBM (or streamed), gen_blcotensor_device."""
 
@@ -445,6 +457,56 @@ class DeviceTensor:
         self.nmodes = host.nmodes
         self.dims = host.dims.copy()
         self.nnz = host.nnz
+        self.value_dtype = value_dtype
+
+    @classmethod
+    def generate(cls, dims, nnz, zipf=None, gen_rank=10, value_kind=0, seed=1, topup=False,
+                 max_block_size=0, value_dtype=F64):
+        """The tensor of Coo.generate(...) encoded and made resident on the GPU
+        without a host copy (blco_generate_device)."""
+        s = _gen_spec(dims, nnz, zipf, gen_rank, value_kind, seed, topup)
+        h = vp()
+        _check(lib().blco_generate_device(C.byref(s), max_block_size, value_dtype, C.byref(h)))
+        self = cls.__new__(cls)
+        self.h = h.value
+        n = C.c_int(0)
+        dims = np.zeros(MAX_MODES, np.uint64)
+        nnz = C.c_uint64(0); nb = C.c_uint64(0)
+        _check(lib().blco_dev_info(self.h, C.byref(n), dims, C.byref(nnz), C.byref(nb)))
+        self.nmodes, self.dims, self.nnz, self.nblocks = n.value, dims[:n.value].copy(), nnz.value, nb.value
+        self.value_dtype = value_dtype
+        return self
+
+    def to_host(self):
+        """Host copy (BlcoHost, values as f64) of a resident tensor."""
+        h = vp()
+        _check(lib().blco_dev_to_host(self.h, C.byref(h)))
+        return BlcoHost(h.value)
+
+    def sample(self, step):
+        """Every step-th nonzero in BLCO order, as a Coo."""
+        h = vp()
+        _check(lib().blco_dev_sample(self.h, int(step), C.byref(h)))
+        return Coo(h.value)
+
+    def row_copy(self, mode, rank, factor_dtype=F32):
+        """Kernel 14's row-ordered copy of `mode` (built if needed): dict of
+        tiles, inner, fibers, coords, vals, runs, groups (n x 4)."""
+        info = np.zeros(5, np.uint64)
+        _check(lib().blco_dev_row_copy(self.h, mode, rank, factor_dtype, info, None, None, None, None))
+        n = int(self.nnz)
+        coords = np.zeros(n, np.uint64)
+        vals = np.zeros(n, np.float64 if self.value_dtype == F64 else np.float32)
+        runs = np.zeros(int(info[4]), np.uint64)
+        groups = np.zeros((int(info[3]), 4), np.uint64)
+        _check(lib().blco_dev_row_copy(self.h, mode, rank, factor_dtype, info, coords.ctypes.data,
+                                       vals.ctypes.data, runs.ctypes.data, groups.ctypes.data))
+        inner = int(info[1])
+        return dict(tiles=int(info[0]), inner=inner - (1 << 64) if inner >= (1 << 63) else inner,
+                    fibers=bool(info[2]), coords=coords, vals=vals, runs=runs, groups=groups)
+
+    def set_value_dtype(self, value_dtype):
+        _check(lib().blco_dev_set_value_dtype(self.h, value_dtype))
         self.value_dtype = value_dtype
 
     def bytes(self):

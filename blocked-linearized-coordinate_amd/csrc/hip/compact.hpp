@@ -115,7 +115,7 @@ int compact_count(uint64_t n, Pred pred, uint64_t* count, CompactScratch* s, hip
     uint64_t* cnt = s->tiles;
     uint64_t* off = s->tiles + tiles + 1;
     size_t need = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, (int)tiles + 1, st));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, tiles + 1, st));
     if (s->tmp_cap < need) {
         if (s->tmp) (void)hipFree(s->tmp);
         s->tmp = nullptr;
@@ -128,7 +128,7 @@ int compact_count(uint64_t n, Pred pred, uint64_t* count, CompactScratch* s, hip
     // one more zero count so that off[tiles] is the total
     HIP_TRY(hipMemsetAsync(cnt + tiles, 0, sizeof(uint64_t), st));
     size_t have = s->tmp_cap;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s->tmp, have, cnt, off, (int)tiles + 1, st));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s->tmp, have, cnt, off, tiles + 1, st));
     HIP_TRY(hipMemcpyAsync(count, off + tiles, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return BLCO_OK;

@@ -398,10 +398,13 @@ constexpr double FIBER_MIN_LEN = 1.25;
 constexpr uint64_t FIBER_MIN_INNER = 2ull << 20;
 constexpr uint64_t FIBER_MIN_INNER_F64 = 64ull << 20;
 
+// Counts behind the decision come from the host copy (coords, bc) or, when
+// coords is empty, from the device tensor.
 static int xcd_tiles(const blco_dev* d, int mode, int split, uint64_t row_bytes,
                      const std::vector<uint64_t>& coords, const std::vector<uint64_t>& bc,
                      std::vector<uint8_t>* tile_of)
 {
+    const bool on_device = coords.empty();
     const char* env = getenv("BLCO_XCD_TILES");
     const bool force = env && env[0] == '1', off = env && env[0] == '0';
     if (off || split < 0) return 1;
@@ -413,10 +416,12 @@ static int xcd_tiles(const blco_dev* d, int mode, int split, uint64_t row_bytes,
         return ((coords[i] >> d->pos[m]) & d->masks[m]) | bc[b * N + m];
     };
     std::vector<uint64_t> cnt(I, 0);
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (uint64_t b = 0; b < d->nblocks; ++b)
-        for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
-            __atomic_fetch_add(&cnt[index(split, b, i)], 1ull, __ATOMIC_RELAXED);
+    if (on_device) {
+        if (dev_split_counts(d, split, &cnt)) return 1;  // no tiling when the counts fail
+    } else {
+        for (uint64_t b = 0; b < d->nblocks; ++b)
+            for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i) ++cnt[index(split, b, i)];
+    }
     tile_of->assign(I, 0);
     uint64_t cum = 0, tile_nnz[NUM_XCD] = {};
     for (uint64_t i = 0; i < I; ++i) {
@@ -431,21 +436,25 @@ static int xcd_tiles(const blco_dev* d, int mode, int split, uint64_t row_bytes,
     // the tiles each output row meets
     std::vector<uint8_t> met(rows, 0);
     const uint8_t* to = tile_of->data();
-    #pragma omp parallel for schedule(dynamic, 1)
-    for (uint64_t b = 0; b < d->nblocks; ++b)
-        for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
-            __atomic_fetch_or(&met[index(mode, b, i)], (uint8_t)(1u << to[index(split, b, i)]), __ATOMIC_RELAXED);
+    if (on_device) {
+        if (dev_row_tiles(d, mode, split, *tile_of, &met)) return 1;
+    } else {
+        for (uint64_t b = 0; b < d->nblocks; ++b)
+            for (uint64_t i = d->bstart[b]; i < d->bstart[b] + d->bnnz[b]; ++i)
+                met[index(mode, b, i)] |= (uint8_t)(1u << to[index(split, b, i)]);
+    }
     uint64_t added = 0;
-    #pragma omp parallel for schedule(static) reduction(+ : added)
     for (uint64_t r = 0; r < rows; ++r)
         if (met[r]) added += (uint64_t)__builtin_popcount(met[r]) - 1;
     return added * XCD_TILE_RUN_RATIO <= n ? NUM_XCD : 1;
 }
 
 // Builds the row-ordered copy of a mode (kernel 14) from the resident
-// tensor: download, stable sort on the host, upload.  Once per mode.  The
-// copy is grouped by (tile of the split mode, block) and sorted by the mode
-// index within a group, where the split mode is the longest other mode: the
+// tensor, once per mode: on the GPU (roworder_dev.hip), or with
+// BLCO_ROWORDER=host by a download, a stable sort on the host and an upload
+// (the same copy; the host build is the reference of the tests).  The copy
+// is grouped by (tile of the split mode, block) and sorted by the mode index
+// within a group, where the split mode is the longest other mode: the
 // chunks of tile t all run on XCD t, whose L2 then holds only 1/NUM_XCD of
 // the split mode's factor rows.
 static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
@@ -454,12 +463,10 @@ static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
     const uint64_t n = d->nnz, nb = d->nblocks;
     const int N = d->nmodes;
     const size_t vsz = value_size(d->vdtype);
-    std::vector<uint64_t> coords(n), perm(n), sc(n), bc(nb * N);
-    std::vector<char> vals(n * vsz), sv(n * vsz);
-    progress("row-ordered copy of mode %d: %llu nonzeros", mode, (unsigned long long)n);
-    HIP_TRY(hipMemcpy(coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(vals.data(), d->d_vals, n * vsz, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(bc.data(), d->d_bcoords, nb * N * 8, hipMemcpyDeviceToHost));
+    const char* renv = getenv("BLCO_ROWORDER");
+    bool on_host = renv && std::strcmp(renv, "host") == 0;
+    progress("row-ordered copy of mode %d: %llu nonzeros (%s)", mode, (unsigned long long)n,
+             on_host ? "host" : "device");
     // fiber order: the inner mode is the longest input mode, the others are
     // outer; the split mode for the XCD tiles is then the longest outer mode
     // (a split of the inner mode would cut the fibers), else (ALTO order
@@ -472,7 +479,6 @@ static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
     o.nblocks = nb;
     o.block_start = d->bstart.data();
     o.block_nnz = d->bnnz.data();
-    o.bcoords = bc.data();
     int inner = -1;
     const char* fenv = getenv("BLCO_FIBERS");
     if (N >= 3 && !(fenv && fenv[0] == '0')) {
@@ -496,76 +502,109 @@ static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
     int split = -1;
     for (int m = 0; m < N; ++m)
         if (m != mode && m != inner && (split < 0 || d->dims[m] > d->dims[split])) split = m;
+    std::vector<uint64_t> coords, bc;
+    std::vector<char> vals;
+    auto download = [&]() -> int {
+        try {
+            coords.resize(n);
+            vals.resize(n * vsz);
+            bc.resize(nb * N);
+        } catch (const std::bad_alloc&) {
+            return fail(BLCO_ENOMEM, "host copy for the row-ordered copy of mode %d", mode);
+        }
+        HIP_TRY(hipMemcpy(coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(vals.data(), d->d_vals, n * vsz, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(bc.data(), d->d_bcoords, nb * N * 8, hipMemcpyDeviceToHost));
+        return BLCO_OK;
+    };
+    if (on_host) {
+        int rc = download();
+        if (rc) return rc;
+    }
+    o.bcoords = on_host ? bc.data() : nullptr;
     std::vector<uint8_t> tile_of;
     const int ntiles = xcd_tiles(d, mode, split, (uint64_t)rank * dtype_size(ftype), coords, bc, &tile_of);
     o.split = split;
     o.ntiles = ntiles;
     o.tile_of = tile_of.data();
-    std::vector<uint64_t> gnnz((uint64_t)ntiles * nb);
-    row_order(o, n, coords.data(), perm.data(), gnnz.data());
-    progress("row-ordered copy of mode %d: sorted (%d tiles of mode %d, inner mode %d)", mode, ntiles, split,
-             inner);
-    gather_bytes(coords.data(), 8, perm.data(), n, sc.data());
-    gather_bytes(vals.data(), vsz, perm.data(), n, sv.data());
-    // groups in storage order, and run starts of equal (group, row)
     std::vector<SGroup> groups;
     std::vector<uint64_t> runs;
-    const uint64_t mk = d->masks[mode];
-    const int ps = d->pos[mode];
-    uint64_t start = 0;
-    for (uint64_t g = 0; g < gnnz.size(); ++g) {
-        if (!gnnz[g]) continue;
-        groups.push_back(SGroup{start, gnnz[g], (uint32_t)(g % nb), (uint32_t)(g / nb)});
-        for (uint64_t i = start; i < start + gnnz[g]; ++i)
-            if (i == start || ((sc[i] >> ps) & mk) != ((sc[i - 1] >> ps) & mk)) runs.push_back(i);
-        start += gnnz[g];
-    }
-    runs.push_back(n);
-    // fibers: positions where the group, the row or an outer index changes
     uint64_t fibers = 0;
-    if (inner >= 0) {
-        uint64_t fm[BLCO_MAX_MODES];
-        int fp[BLCO_MAX_MODES], nf = 0;
-        fm[nf] = mk;
-        fp[nf++] = ps;
-        for (int j = 0; j < o.nouter; ++j) {
-            fm[nf] = d->masks[o.outer[j]];
-            fp[nf++] = d->pos[o.outer[j]];
-        }
-        const int64_t ng = (int64_t)groups.size();
-        #pragma omp parallel for schedule(dynamic, 1) reduction(+ : fibers)
-        for (int64_t gi = 0; gi < ng; ++gi) {
-            const uint64_t lo = groups[gi].start, hi = lo + groups[gi].nnz;
-            uint64_t f = 1;
-            for (uint64_t i = lo + 1; i < hi; ++i) {
-                bool first = false;
-                for (int k = 0; k < nf && !first; ++k)
-                    first = ((sc[i] >> fp[k]) & fm[k]) != ((sc[i - 1] >> fp[k]) & fm[k]);
-                f += first;
-            }
-            fibers += f;
+    uint64_t* dc = nullptr;
+    void* dv = nullptr;
+    if (!on_host) {
+        const int rc = dev_row_copy(d, o, &groups, &runs, inner >= 0 ? &fibers : nullptr, &dc, &dv);
+        if (rc == BLCO_EUNSUPPORTED) {  // key wider than 64 bits: the host build
+            on_host = true;
+            int r2 = download();
+            if (r2) return r2;
+            o.bcoords = bc.data();
+        } else if (rc) {
+            return rc;
         }
     }
+    if (on_host) {
+        std::vector<uint64_t> perm(n), sc(n);
+        std::vector<char> sv(n * vsz);
+        std::vector<uint64_t> gnnz((uint64_t)ntiles * nb);
+        row_order(o, n, coords.data(), perm.data(), gnnz.data());
+        gather_bytes(coords.data(), 8, perm.data(), n, sc.data());
+        gather_bytes(vals.data(), vsz, perm.data(), n, sv.data());
+        // groups in storage order, and run starts of equal (group, row)
+        const uint64_t mk = d->masks[mode];
+        const int ps = d->pos[mode];
+        uint64_t start = 0;
+        for (uint64_t g = 0; g < gnnz.size(); ++g) {
+            if (!gnnz[g]) continue;
+            groups.push_back(SGroup{start, gnnz[g], (uint32_t)(g % nb), (uint32_t)(g / nb)});
+            for (uint64_t i = start; i < start + gnnz[g]; ++i)
+                if (i == start || ((sc[i] >> ps) & mk) != ((sc[i - 1] >> ps) & mk)) runs.push_back(i);
+            start += gnnz[g];
+        }
+        runs.push_back(n);
+        // fibers: positions where the group, the row or an outer index changes
+        if (inner >= 0) {
+            uint64_t fm[BLCO_MAX_MODES];
+            int fp[BLCO_MAX_MODES], nf = 0;
+            fm[nf] = mk;
+            fp[nf++] = ps;
+            for (int j = 0; j < o.nouter; ++j) {
+                fm[nf] = d->masks[o.outer[j]];
+                fp[nf++] = d->pos[o.outer[j]];
+            }
+            for (const SGroup& g : groups) {
+                const uint64_t lo = g.start, hi = lo + g.nnz;
+                uint64_t f = 1;
+                for (uint64_t i = lo + 1; i < hi; ++i) {
+                    bool first = false;
+                    for (int k = 0; k < nf && !first; ++k)
+                        first = ((sc[i] >> fp[k]) & fm[k]) != ((sc[i - 1] >> fp[k]) & fm[k]);
+                    f += first;
+                }
+                fibers += f;
+            }
+        }
+        auto release = [&]() {
+            if (dc) (void)hipFree(dc);
+            if (dv) (void)hipFree(dv);
+        };
+        if (hipMalloc(&dc, n * 8) != hipSuccess || hipMalloc(&dv, n * vsz) != hipSuccess) {
+            release();
+            return fail(BLCO_ENOMEM, "no device memory for the row-ordered copy of mode %d (%llu bytes)", mode,
+                        (unsigned long long)(n * (8 + vsz)));
+        }
+        if (hipMemcpy(dc, sc.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dv, sv.data(), n * vsz, hipMemcpyHostToDevice) != hipSuccess) {
+            release();
+            return fail(BLCO_EHIP, "upload of the row-ordered copy failed");
+        }
+    }
+    progress("row-ordered copy of mode %d: sorted (%d tiles of mode %d, inner mode %d)", mode, ntiles, split,
+             inner);
     const bool use_fibers = inner >= 0 && ((fenv && fenv[0] == '1') || (double)n >= FIBER_MIN_LEN * (double)fibers);
     if (inner >= 0)
         progress("row-ordered copy of mode %d: %.2f nonzeros per fiber%s", mode, (double)n / (double)fibers,
                  use_fibers ? "" : " (fiber kernel off)");
-    uint64_t* dc = nullptr;
-    void* dv = nullptr;
-    auto release = [&]() {
-        if (dc) (void)hipFree(dc);
-        if (dv) (void)hipFree(dv);
-    };
-    if (hipMalloc(&dc, n * 8) != hipSuccess || hipMalloc(&dv, n * vsz) != hipSuccess) {
-        release();
-        return fail(BLCO_ENOMEM, "no device memory for the row-ordered copy of mode %d (%llu bytes)", mode,
-                    (unsigned long long)(n * (8 + vsz)));
-    }
-    if (hipMemcpy(dc, sc.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dv, sv.data(), n * vsz, hipMemcpyHostToDevice) != hipSuccess) {
-        release();
-        return fail(BLCO_EHIP, "upload of the row-ordered copy failed");
-    }
     d->d_scoords[mode] = dc;
     d->d_svals[mode] = dv;
     d->bytes += n * (8 + vsz);
@@ -1181,6 +1220,188 @@ int blco_dev_create(const blco_host* h, int value_dtype, int stream_data, uint64
                                   h->coords.data(), h->vals.data(), h->block_nnz.size(),
                                   h->block_start.data(), h->block_nnz.data(),
                                   h->block_coords.data(), value_dtype, stream_data, slot_nnz, out);
+}
+
+int blco_dev_row_copy(blco_dev* d, int mode, int rank, int factor_dtype, uint64_t* info, uint64_t* coords,
+                      void* vals, uint64_t* runs, uint64_t* groups)
+{
+    if (!d || !info || mode < 0 || mode >= d->nmodes || rank < 1) return fail(BLCO_EINVAL, "bad row-copy arguments");
+    if (d->stream_data) return fail(BLCO_EINVAL, "a streamed tensor has no row-ordered copies");
+    if (factor_dtype < BLCO_F64 || factor_dtype > BLCO_BF16) return fail(BLCO_EINVAL, "factor dtype %d", factor_dtype);
+    int rc = ensure_row_ordered(d, mode, rank, factor_dtype);
+    if (rc) return rc;
+    const uint64_t n = d->nnz;
+    info[0] = (uint64_t)d->stiles[mode];
+    info[1] = (uint64_t)(int64_t)d->sinner[mode];
+    info[2] = d->sfiber[mode] ? 1 : 0;
+    info[3] = d->sgrp_h[mode].size();
+    info[4] = d->srun_h[mode].size();
+    if (n && coords) HIP_TRY(hipMemcpy(coords, d->d_scoords[mode], n * 8, hipMemcpyDeviceToHost));
+    if (n && vals) HIP_TRY(hipMemcpy(vals, d->d_svals[mode], n * value_size(d->vdtype), hipMemcpyDeviceToHost));
+    if (runs) std::memcpy(runs, d->srun_h[mode].data(), d->srun_h[mode].size() * 8);
+    if (groups)
+        for (size_t g = 0; g < d->sgrp_h[mode].size(); ++g) {
+            const SGroup& x = d->sgrp_h[mode][g];
+            groups[4 * g] = x.start;
+            groups[4 * g + 1] = x.nnz;
+            groups[4 * g + 2] = x.block;
+            groups[4 * g + 3] = x.tile;
+        }
+    return BLCO_OK;
+}
+
+int blco_dev_info(const blco_dev* d, int* nmodes, uint64_t* dims, uint64_t* nnz, uint64_t* nblocks)
+{
+    if (!d) return fail(BLCO_EINVAL, "null device tensor");
+    if (nmodes) *nmodes = d->nmodes;
+    if (dims)
+        for (int m = 0; m < d->nmodes; ++m) dims[m] = d->dims[m];
+    if (nnz) *nnz = d->nnz;
+    if (nblocks) *nblocks = d->nblocks;
+    return BLCO_OK;
+}
+
+int blco_dev_to_host(const blco_dev* d, blco_host** out)
+{
+    if (!d || !out) return fail(BLCO_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->stream_data) return fail(BLCO_EINVAL, "a streamed tensor is already in host memory");
+    blco_host* h = new (std::nothrow) blco_host();
+    if (!h) return fail(BLCO_ENOMEM, "out of host memory");
+    int rc = make_layout(d->nmodes, d->dims, &h->L);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    const uint64_t n = d->nnz, nb = d->nblocks;
+    try {
+        h->coords.resize(n);
+        h->vals.resize(n);
+        h->block_coords.resize(nb * d->nmodes);
+    } catch (const std::bad_alloc&) {
+        delete h;
+        return fail(BLCO_ENOMEM, "host copy of %llu nonzeros", (unsigned long long)n);
+    }
+    h->nnz = n;
+    h->block_start = d->bstart;
+    h->block_nnz = d->bnnz;
+    for (uint64_t b = 0; b < nb; ++b) h->max_block_nnz = std::max(h->max_block_nnz, d->bnnz[b]);
+    bool ok = (n == 0 || hipMemcpy(h->coords.data(), d->d_coords, n * 8, hipMemcpyDeviceToHost) == hipSuccess) &&
+              (nb == 0 || hipMemcpy(h->block_coords.data(), d->d_bcoords, nb * d->nmodes * 8,
+                                    hipMemcpyDeviceToHost) == hipSuccess);
+    if (ok && n) {
+        if (d->vdtype == BLCO_F64) {
+            ok = hipMemcpy(h->vals.data(), d->d_vals, n * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        } else {
+            std::vector<float> f(n);
+            ok = hipMemcpy(f.data(), d->d_vals, n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+            for (uint64_t i = 0; ok && i < n; ++i) h->vals[i] = f[i];
+        }
+    }
+    if (!ok) {
+        delete h;
+        return fail(BLCO_EHIP, "download of the device tensor failed");
+    }
+    *out = h;
+    return BLCO_OK;
+}
+
+int blco_dev_sample(const blco_dev* d, uint64_t step, blco_coo** out)
+{
+    if (!d || !out || step == 0) return fail(BLCO_EINVAL, "bad sample arguments");
+    *out = nullptr;
+    if (d->stream_data) return fail(BLCO_EINVAL, "sampling takes a resident tensor");
+    const uint64_t n = d->nnz, cnt = (n + step - 1) / step;
+    const int N = d->nmodes;
+    std::vector<uint64_t> c(cnt), bc(d->nblocks * N);
+    std::vector<double> v(cnt);
+    const size_t vsz = value_size(d->vdtype);
+    std::vector<char> vb(cnt * vsz);
+    if (cnt) {
+        HIP_TRY(hipMemcpy2D(c.data(), 8, d->d_coords, step * 8, 8, cnt, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy2D(vb.data(), vsz, d->d_vals, step * vsz, vsz, cnt, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(bc.data(), d->d_bcoords, bc.size() * 8, hipMemcpyDeviceToHost));
+    }
+    blco_coo* t = new (std::nothrow) blco_coo();
+    if (!t) return fail(BLCO_ENOMEM, "out of host memory");
+    t->nmodes = N;
+    for (int m = 0; m < N; ++m) t->dims[m] = d->dims[m];
+    t->nnz = cnt;
+    t->cidx.resize((uint64_t)N * cnt);
+    t->vals.resize(cnt);
+    uint64_t b = 0;
+    for (uint64_t k = 0; k < cnt; ++k) {
+        const uint64_t i = k * step;
+        while (b + 1 < d->nblocks && d->bstart[b + 1] <= i) ++b;
+        for (int m = 0; m < N; ++m)
+            t->cidx[(uint64_t)m * cnt + k] = ((c[k] >> d->pos[m]) & d->masks[m]) | bc[b * N + m];
+        t->vals[k] = vsz == 8 ? reinterpret_cast<const double*>(vb.data())[k]
+                              : (double)reinterpret_cast<const float*>(vb.data())[k];
+    }
+    *out = t;
+    return BLCO_OK;
+}
+
+__global__ void k_vals_to_f32(uint64_t n, const double* __restrict__ src, float* __restrict__ dst)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG)
+        dst[i] = (float)src[i];
+}
+
+__global__ void k_vals_to_f64(uint64_t n, const float* __restrict__ src, double* __restrict__ dst)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG)
+        dst[i] = (double)src[i];
+}
+
+int blco_dev_set_value_dtype(blco_dev* d, int value_dtype)
+{
+    if (!d) return fail(BLCO_EINVAL, "null device tensor");
+    if (value_dtype != BLCO_F64 && value_dtype != BLCO_F32) return fail(BLCO_EINVAL, "value dtype %d", value_dtype);
+    if (value_dtype == d->vdtype) return BLCO_OK;
+    if (d->stream_data) return fail(BLCO_EINVAL, "the value type of a streamed tensor is fixed");
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    const uint64_t n = d->nnz;
+    const size_t old_sz = value_size(d->vdtype), new_sz = value_size(value_dtype);
+    if (n) {
+        void* nv = nullptr;
+        if (hipMalloc(&nv, n * new_sz) != hipSuccess)
+            return fail(BLCO_ENOMEM, "values of %llu nonzeros", (unsigned long long)n);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((n + WG - 1) / WG, 65536);
+        if (value_dtype == BLCO_F32)
+            hipLaunchKernelGGL(k_vals_to_f32, dim3(grid), dim3(WG), 0, d->stream, n, (const double*)d->d_vals, (float*)nv);
+        else
+            hipLaunchKernelGGL(k_vals_to_f64, dim3(grid), dim3(WG), 0, d->stream, n, (const float*)d->d_vals, (double*)nv);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(d->stream) != hipSuccess) {
+            (void)hipFree(nv);
+            return fail(BLCO_EHIP, "value conversion failed");
+        }
+        (void)hipFree(d->d_vals);
+        d->d_vals = nv;
+        d->bytes = d->bytes - n * old_sz + n * new_sz;
+    }
+    // the row-ordered copies hold values of the old type: rebuilt on next use
+    for (int m = 0; m < BLCO_MAX_MODES; ++m) {
+        if (d->d_scoords[m]) {
+            (void)hipFree(d->d_scoords[m]);
+            d->d_scoords[m] = nullptr;
+            d->bytes -= n * (8 + old_sz);
+        }
+        if (d->d_svals[m]) (void)hipFree(d->d_svals[m]);
+        d->d_svals[m] = nullptr;
+        for (auto& t : d->stabs[m])
+            if (t.d) (void)hipFree(t.d);
+        d->stabs[m].clear();
+        d->sgrp_h[m].clear();
+        d->srun_h[m].clear();
+        d->sorted_failed[m] = false;
+        d->stiles[m] = 0;
+        d->sinner[m] = 0;
+        d->sfiber[m] = false;
+    }
+    d->sel_cache.clear();
+    d->vdtype = value_dtype;
+    return BLCO_OK;
 }
 
 void blco_dev_free(blco_dev* d)

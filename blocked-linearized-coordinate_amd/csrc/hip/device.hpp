@@ -9,6 +9,7 @@
 
 #include "blco_hip.h"
 #include "mttkrp.hpp"
+#include "../host/internal.hpp"
 
 namespace blco {
 
@@ -138,6 +139,17 @@ struct blco_dev {
 namespace blco {
 
 const ChunkTab* chunk_table(blco_dev* d, int tpw);
+// Row-ordered copies built on the GPU (roworder_dev.hip): nonzeros per
+// split-mode index; per output row the tiles (bit t for tile t) its nonzeros
+// meet; and the copy itself in the order of row_order (csrc/host), with its
+// groups in storage order, the run starts of (group, row) followed by nnz,
+// and the fiber count when fibers is non-null.  BLCO_EUNSUPPORTED when the
+// sort key would exceed 64 bits.
+int dev_split_counts(const blco_dev* d, int split, std::vector<uint64_t>* cnt);
+int dev_row_tiles(const blco_dev* d, int mode, int split, const std::vector<uint8_t>& tile_of,
+                  std::vector<uint8_t>* met);
+int dev_row_copy(const blco_dev* d, const RowOrderSpec& o, std::vector<SGroup>* groups, std::vector<uint64_t>* runs,
+                 uint64_t* fibers, uint64_t** sc, void** sv);
 const WindowTab* window_table(blco_dev* d, int k);
 const WindowPlan* window_plan(blco_dev* d, int mode, int rank, int acc_bytes);
 int auto_kernel(const blco_dev* d, int mode, int rank, int ftype);

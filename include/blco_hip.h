@@ -105,6 +105,27 @@ uint64_t blco_dev_bytes(const blco_dev* d);
 int blco_dev_arrays(const blco_dev* d, uint64_t** coords, void** vals, int* value_dtype,
                     uint64_t** block_coords);
 
+/* The synthetic Poisson tensor of blco_generate (blco_host.h) generated,
+ * encoded (gen_blcotensor_host semantics, blocks of at most max_block_size
+ * nonzeros per upper key, 0 = unbounded) and made resident on the GPU
+ * without a host copy of the tensor: the same tensor as blco_generate +
+ * blco_host_encode + blco_dev_create(value_dtype, resident) (no reference
+ * counterpart: the reference generates on the host,
+ * src/poisson_generator.cpp).  Keys of at most 72 bits. */
+int blco_generate_device(const blco_gen_spec* spec, uint64_t max_block_size,
+                         int value_dtype, blco_dev** out);
+/* Shape of a device tensor: nmodes, dims (nmodes entries), nonzeros and
+ * blocks (any pointer may be NULL). */
+int blco_dev_info(const blco_dev* d, int* nmodes, uint64_t* dims, uint64_t* nnz, uint64_t* nblocks);
+/* Host copy of a resident device tensor (values as f64). */
+int blco_dev_to_host(const blco_dev* d, blco_host** out);
+/* Every step-th nonzero (in BLCO order) of a resident tensor, decoded to COO
+ * on the host: a uniform sample for CPU-side baselines. */
+int blco_dev_sample(const blco_dev* d, uint64_t step, blco_coo** out);
+/* Converts the resident values to value_dtype (BLCO_F64 or BLCO_F32) in
+ * place; row-ordered copies are dropped (rebuilt on their next use). */
+int blco_dev_set_value_dtype(blco_dev* d, int value_dtype);
+
 /* ----------------------------------------------------------------- MTTKRP */
 
 /* mttkrp_alto_dev_onemode, src/alto_dev.cu:1954-2032:
@@ -117,6 +138,16 @@ int blco_dev_arrays(const blco_dev* d, uint64_t** coords, void** vals, int* valu
 int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype,
                 void* const* factors, void* out, int kernel, int nprtn,
                 int thread_cf, void* stream);
+
+/* The row-ordered copy behind kernel 14 for (mode, rank, factor_dtype),
+ * built if it is not (on the GPU; with BLCO_ROWORDER=host in the
+ * environment on the host), for inspection: info[5] = tiles, inner mode (-1:
+ * ALTO order within rows), fiber kernel on (0/1), groups, run starts (with
+ * the final nnz); then (each may be NULL) its BLCO words (nnz), values
+ * (nnz, the tensor's value type), run starts (info[4]) and groups (4 per
+ * group: first nonzero, nonzeros, block, tile). */
+int blco_dev_row_copy(blco_dev* d, int mode, int rank, int factor_dtype, uint64_t* info,
+                      uint64_t* coords, void* vals, uint64_t* runs, uint64_t* groups);
 
 /* Kernel the AUTO heuristic would run for (mode, rank)
  * (src/alto_dev.cu:1975,1996-2001). */
