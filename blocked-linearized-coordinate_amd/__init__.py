@@ -49,7 +49,7 @@ class CpalsOpts(C.Structure):
     _fields_ = [("rank", C.c_int), ("max_iters", C.c_int), ("tolerance", C.c_double),
                 ("kernel", C.c_int), ("precision", C.c_int), ("nprtn", C.c_int),
                 ("thread_cf", C.c_int), ("mode_order", C.c_int), ("use_graph", C.c_int),
-                ("verbose", C.c_int)]
+                ("verbose", C.c_int), ("solve_floor", C.c_int)]
 
 
 def build():
@@ -754,12 +754,13 @@ def fit(grams, rank, lam, U_last, M_last, dtype, rows, normX2, stream=None):
 
 
 def _opts(rank, max_iters, tolerance, kernel, precision, nprtn, thread_cf, mode_order,
-          use_graph, verbose):
+          use_graph, verbose, solve_floor=0):
     o = CpalsOpts()
     o.rank = rank; o.max_iters = max_iters; o.tolerance = tolerance; o.kernel = kernel
     o.precision = precision; o.nprtn = nprtn; o.thread_cf = thread_cf
     o.mode_order = mode_order; o.use_graph = 1 if use_graph else 0
     o.verbose = 1 if verbose else 0
+    o.solve_floor = solve_floor
     return o
 
 
@@ -767,12 +768,12 @@ class CpAls:
     """Device-resident CP-ALS solver (cpals_blco_dev split into steps)."""
 
     def __init__(self, dev, U0, rank, precision=F64, kernel=K_AUTO, tolerance=0.0, nprtn=0,
-                 thread_cf=0, mode_order=0, use_graph=False, verbose=False):
+                 thread_cf=0, mode_order=0, use_graph=False, verbose=False, solve_floor=0):
         self.dev = dev
         self.rank = rank
         self.nmodes = dev.nmodes
         self.o = _opts(rank, 0, tolerance, kernel, precision, nprtn, thread_cf, mode_order,
-                       use_graph, verbose)
+                       use_graph, verbose, solve_floor)
         U0 = [np.ascontiguousarray(u, np.float64) for u in U0]
         h = vp()
         _check(lib().blco_cpals_create(dev.h, C.byref(self.o),

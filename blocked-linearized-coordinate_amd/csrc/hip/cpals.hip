@@ -72,7 +72,9 @@ int enqueue_factorize(blco_cpals_state* s, int n, hipStream_t st)
 {
     GramSet gs;
     for (int m = 0; m < s->N; ++m) gs.g[m] = s->G[m];
-    return launch_factorize(gs, s->N, n, s->R, gram_eps(s->ft), s->fac, s->work, s->status, st);
+    // the pivot floor of f64 unless the caller asked for the factors' precision
+    const double eps = s->o.solve_floor ? gram_eps(s->ft) : gram_eps(BLCO_F64);
+    return launch_factorize(gs, s->N, n, s->R, eps, s->fac, s->work, s->status, st);
 }
 
 int enqueue_step(blco_cpals_state* s, int n, int which_norm, hipStream_t st)

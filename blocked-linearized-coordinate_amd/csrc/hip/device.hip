@@ -1306,6 +1306,17 @@ int blco_dev_to_host(const blco_dev* d, blco_host** out)
     return BLCO_OK;
 }
 
+__global__ void k_sample_gather(uint64_t cnt, uint64_t step, int vsz, const uint64_t* __restrict__ coords,
+                                const void* __restrict__ vals, uint64_t* __restrict__ gc, void* __restrict__ gv)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * WG + threadIdx.x; k < cnt; k += (uint64_t)gridDim.x * WG) {
+        const uint64_t i = k * step;
+        gc[k] = coords[i];
+        if (vsz == 8) static_cast<double*>(gv)[k] = static_cast<const double*>(vals)[i];
+        else static_cast<float*>(gv)[k] = static_cast<const float*>(vals)[i];
+    }
+}
+
 int blco_dev_sample(const blco_dev* d, uint64_t step, blco_coo** out)
 {
     if (!d || !out || step == 0) return fail(BLCO_EINVAL, "bad sample arguments");
@@ -1318,9 +1329,23 @@ int blco_dev_sample(const blco_dev* d, uint64_t step, blco_coo** out)
     const size_t vsz = value_size(d->vdtype);
     std::vector<char> vb(cnt * vsz);
     if (cnt) {
-        HIP_TRY(hipMemcpy2D(c.data(), 8, d->d_coords, step * 8, 8, cnt, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy2D(vb.data(), vsz, d->d_vals, step * vsz, vsz, cnt, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(bc.data(), d->d_bcoords, bc.size() * 8, hipMemcpyDeviceToHost));
+        // gathered on the device into contiguous buffers, then one copy each
+        uint64_t* gc = nullptr;
+        void* gv = nullptr;
+        if (hipMalloc(&gc, cnt * 8) != hipSuccess || hipMalloc(&gv, cnt * vsz) != hipSuccess) {
+            if (gc) (void)hipFree(gc);
+            return fail(BLCO_ENOMEM, "sample buffers of %llu nonzeros", (unsigned long long)cnt);
+        }
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((cnt + WG - 1) / WG, 65536);
+        hipLaunchKernelGGL(k_sample_gather, dim3(grid), dim3(WG), 0, 0, cnt, step, (int)vsz, d->d_coords,
+                           d->d_vals, gc, gv);
+        const bool ok = hipGetLastError() == hipSuccess &&
+                        hipMemcpy(c.data(), gc, cnt * 8, hipMemcpyDeviceToHost) == hipSuccess &&
+                        hipMemcpy(vb.data(), gv, cnt * vsz, hipMemcpyDeviceToHost) == hipSuccess &&
+                        hipMemcpy(bc.data(), d->d_bcoords, bc.size() * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        (void)hipFree(gc);
+        (void)hipFree(gv);
+        if (!ok) return fail(BLCO_EHIP, "sample download failed");
     }
     blco_coo* t = new (std::nothrow) blco_coo();
     if (!t) return fail(BLCO_ENOMEM, "out of host memory");
@@ -1566,8 +1591,7 @@ int blco_solve(double* const* grams, int nmodes, int mode, int rank, const void*
         release();
         return fail(BLCO_ENOMEM, "solve work allocation failed");
     }
-    // the Grams of f32 / bf16 factors carry f32 precision (gram_eps)
-    int rc = launch_factorize(gs, nmodes, mode, rank, gram_eps(dtype), fac, work, status, st);
+    int rc = launch_factorize(gs, nmodes, mode, rank, gram_eps(BLCO_F64), fac, work, status, st);
     if (!rc) rc = launch_solve_rows(M, U, dtype, rows, rank, fac, 0, stats, st);
     int h_status = 0;
     hipError_t e = hipMemcpyAsync(&h_status, status, sizeof(int), hipMemcpyDeviceToHost, st);

@@ -96,7 +96,7 @@ extern "C" int blco_solve_noalloc(double* const* grams, int nmodes, int mode, in
     hipStream_t st = (hipStream_t)stream;
     GramSet gs;
     for (int m = 0; m < nmodes; ++m) gs.g[m] = grams[m];
-    rc = launch_factorize(gs, nmodes, mode, rank, gram_eps(dtype), fac, work, status, st);
+    rc = launch_factorize(gs, nmodes, mode, rank, gram_eps(BLCO_F64), fac, work, status, st);
     if (!rc) rc = launch_solve_rows(M, U, dtype, rows, rank, fac, 0, stats, st);
     if (rc) return rc;
     if (used_fallback) {

@@ -174,8 +174,7 @@ int blco_vecsum(double* x, uint64_t n, void* stream);
  * rank x rank); U = M V^{-1} row by row (Cholesky, forward/backward
  * substitution), or M V^+ when a Cholesky pivot falls to
  * rank * eps * max V_ii or below (symmetric pseudo-inverse, tolerance
- * rank * ulp(max |eig|)); eps and ulp are f64's for dtype BLCO_F64 and f32's
- * for BLCO_F32 / BLCO_BF16 factors, whose Grams carry f32 precision.  M
+ * rank * ulp(max |eig|)), eps and ulp of f64 for every dtype.  M
  * (rows x rank, f64 for dtype BLCO_F64, else f32) is read, U (dtype) written.
  * rank <= 64.  *used_fallback (host, may be NULL) receives 1 when the
  * pseudo-inverse was used (this reads back one flag and synchronises the
@@ -238,6 +237,12 @@ typedef struct blco_cpals_opts {
     int mode_order;     /* 0: modes 0..N-1 (CPU reference), 1: longest first (GPU reference) */
     int use_graph;      /* 1: capture one ALS sweep in a hipGraph and replay it */
     int verbose;        /* 1: print the reference's per-iteration lines */
+    int solve_floor;    /* 0 (default): the solve's pivot floor and pseudo-inverse
+                           tolerance at f64 precision for every form (the FP64
+                           oracle's decisions); 1: at the factors' precision
+                           (f32's for BLCO_F32 / BLCO_BF16), so a V singular to
+                           that precision takes the pseudo-inverse whatever the
+                           MTTKRP's summation order */
 } blco_cpals_opts;
 
 typedef struct blco_cpals_state blco_cpals_state; /* device-resident solver state */

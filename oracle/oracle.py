@@ -79,6 +79,9 @@ def ref_lib():
         R.ref_mttkrp_alto_par.argtypes = [C.c_int, u64p, C.c_uint64, u64p, f64p, C.c_int,
                                           C.c_int, C.POINTER(C.c_void_p), C.c_int, f64p]
         R.ref_kruskal_norm.argtypes = [C.c_uint64, C.c_int, f64p, f64p, C.c_int]
+        R.ref_mttkrp_alto_par_time.argtypes = [C.c_int, u64p, C.c_uint64, u64p, f64p, C.c_int, C.c_int,
+                                               C.POINTER(C.c_void_p), C.c_int, f64p, C.c_int]
+        R.ref_mttkrp_alto_par_time.restype = C.c_double
         _REF = R
     return _REF
 
@@ -269,6 +272,20 @@ def ref_mttkrp(dims, cidx, vals, U, mode, nprtn):
                           np.ascontiguousarray(vals, np.float64), nprtn, rank,
                           _ptrs(Uc), mode, out.reshape(-1))
     return out
+
+
+def ref_mttkrp_time(dims, cidx, vals, U, mode, nprtn, reps=2):
+    """Mean seconds of the reference's own mttkrp_alto_par (compiled from its
+    sources in oracle/_ref) on this tensor, construction excluded."""
+    R = ref_lib()
+    dims = np.ascontiguousarray(dims, np.uint64)
+    cidx = np.ascontiguousarray(cidx, np.uint64)
+    Uc = [np.ascontiguousarray(u, np.float64) for u in U]
+    out = np.zeros((int(dims[mode]), Uc[0].shape[1]), np.float64)
+    ptrs = (C.c_void_p * len(Uc))(*[u.ctypes.data for u in Uc])
+    return R.ref_mttkrp_alto_par_time(len(dims), dims, cidx.shape[1], cidx.ravel(),
+                                      np.ascontiguousarray(vals, np.float64), nprtn, Uc[0].shape[1], ptrs,
+                                      mode, out.ravel(), reps)
 
 
 def ref_kruskal_norm(U, which):

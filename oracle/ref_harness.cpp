@@ -49,6 +49,32 @@ static void free_spt(SparseTensor* X)
     free(X->cidx); free(X->dims); free(X->vals); free(X);
 }
 
+// Mean seconds of one mttkrp_alto_par call of `mode` (omp_get_wtime around
+// each of `reps` calls after one untimed call; ALTO construction and the
+// partial matrices excluded, as src/main.cpp:538-552 times it).  Keys of more
+// than 64 bits take the reference's 128-bit ALTO index.
+template <typename LIT>
+static double time_mttkrp(SparseTensor* X, int nprtn, int rank, double* const* U, int mode, double* out, int reps)
+{
+    AltoTensor<LIT>* at;
+    create_alto(X, &at, nprtn);
+    FType** ofibs = NULL;
+    create_da_mem(mode, (IType)rank, at, &ofibs);
+    FType* factors[MAX_NUM_MODES];
+    for (int n = 0; n < X->nmodes; ++n) factors[n] = U[n];
+    factors[mode] = out;
+    mttkrp_alto_par(mode, factors, (IType)rank, at, NULL, ofibs);
+    double t = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        const double t0 = omp_get_wtime();
+        mttkrp_alto_par(mode, factors, (IType)rank, at, NULL, ofibs);
+        t += omp_get_wtime() - t0;
+    }
+    destroy_da_mem(at, ofibs, (IType)rank, mode);
+    destroy_alto(at);
+    return reps > 0 ? t / reps : 0.0;
+}
+
 extern "C" {
 
 int ref_alto_setup(int nmodes, const uint64_t* dims, int lit_bits,
@@ -111,6 +137,20 @@ int ref_mttkrp_alto_par(int nmodes, const uint64_t* dims, uint64_t nnz,
     destroy_alto(at);
     free_spt(X);
     return 0;
+}
+
+double ref_mttkrp_alto_par_time(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64_t* cidx,
+                                const double* vals, int nprtn, int rank, double* const* U, int mode, double* out,
+                                int reps)
+{
+    SparseTensor* X = make_spt(nmodes, dims, nnz, cidx, vals);
+    int bits = 0;
+    for (int n = 0; n < nmodes; ++n)
+        for (uint64_t d = dims[n] > 1 ? dims[n] - 1 : 0; d; d >>= 1) ++bits;
+    const double t = bits > 64 ? time_mttkrp<unsigned __int128>(X, nprtn, rank, U, mode, out, reps)
+                               : time_mttkrp<unsigned long long>(X, nprtn, rank, U, mode, out, reps);
+    free_spt(X);
+    return t;
 }
 
 // KruskalModelNorm on one factor (which: 0 = MAT_NORM_2, 1 = MAT_NORM_MAX).
