@@ -129,6 +129,18 @@ def summarize(r, dims, nnz, rank, s_v, s_f):
     return r
 
 
+def copy_calibration(B, nbytes):
+    """One device-to-device copy of nbytes (the runtime's copy kernel): the
+    reference for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes."""
+    src = B.DeviceArray((nbytes // 8,), B.F64)
+    dst = B.DeviceArray((nbytes // 8,), B.F64)
+    B._check(B.lib().blco_memcpy_dtod(dst.ptr, src.ptr, src.nbytes))
+    B.synchronize()
+    log("copy calibration: %d bytes device to device" % src.nbytes)
+    src.free()
+    dst.free()
+
+
 def mttkrp_outputs(B, dev, U0, rank, ftype):
     """MTTKRP of every mode from the factors U0 (host f64, converted to
     ftype), copied back as f64."""
@@ -235,6 +247,8 @@ def run_large(B, args):
     log("generated, encoded and resident on the GPU: %d nonzeros, %d blocks, %.1f s, %.2f GB"
         % (nnz, dev.nblocks, gen_s, dev.bytes() / 1e9))
     U0 = B.init_factors(W["seed"], dims, R)
+    if args.copy_calibration:
+        copy_calibration(B, args.copy_calibration)
     forms = [f for f in (args.large_forms or "fp64,fp32").split(",") if f]
     res = {}
     out64 = None
@@ -422,12 +436,7 @@ def run_workload(B, args):
     U0 = B.init_factors(W["seed"], dims, R)
     nnz = coo.nnz
     if args.copy_calibration:
-        src = B.DeviceArray((args.copy_calibration // 8,), B.F64)
-        dst = B.DeviceArray((args.copy_calibration // 8,), B.F64)
-        B._check(B.lib().blco_memcpy_dtod(dst.ptr, src.ptr, src.nbytes))
-        log("copy calibration: %d bytes device to device" % src.nbytes)
-        src.free()
-        dst.free()
+        copy_calibration(B, args.copy_calibration)
     results = {}
     forms = [("fp32", B.F32, 1)] + ([] if args.no_fp64 else [("fp64", B.F64, 0)])
     errs = None

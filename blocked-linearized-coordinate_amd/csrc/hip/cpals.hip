@@ -277,6 +277,14 @@ int blco_cpals_iterate(blco_cpals_state* s, int n_iters, double* fits, int* done
         const int which = it == 0 ? 0 : 1;  // L2 at iteration 0, max norm after
         const bool use_graph = s->o.use_graph && it > 0;
         const double t0 = now_ms();
+        if (it == 0) {
+            // kernel 14's row-ordered copies of every mode, before any MTTKRP
+            // of the sweep runs (their construction would share the GPU with it)
+            for (int k = 0; k < s->N; ++k) {
+                int rc = mttkrp_prepare(s->d, s->order[k], s->R, s->ft, s->o.kernel);
+                if (rc) return rc;
+            }
+        }
         HIP_TRY(hipEventRecord(s->it0, s->st));
         if (use_graph) {
             if (!s->gexec) {

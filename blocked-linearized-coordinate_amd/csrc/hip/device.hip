@@ -710,6 +710,22 @@ static int launch_range(blco_dev* d, int which, MttkrpArgs a, const ChunkTab* t,
     return launch_mttkrp(which, a, ftype, grid, lds, st);
 }
 
+int mttkrp_prepare(blco_dev* d, int mode, int rank, int ftype, int kernel)
+{
+    if (!d || mode < 0 || mode >= d->nmodes || !d->nnz || d->stream_data) return BLCO_OK;
+    int np = 0;
+    if (kernel == BLCO_K_SELECT && !select_kernel(d, mode, rank, ftype, &kernel, &np)) kernel = BLCO_K_AUTO;
+    const bool chosen = kernel == BLCO_K_AUTO;
+    if (chosen) kernel = auto_kernel(d, mode, rank, ftype);
+    if (kernel != BLCO_K_SORTED) return BLCO_OK;
+    const int rc = ensure_row_ordered(d, mode, rank, ftype);
+    if (rc == BLCO_ENOMEM && chosen) {  // no room for the copy: mttkrp() runs the row window
+        d->sorted_failed[mode] = true;
+        return BLCO_OK;
+    }
+    return rc;
+}
+
 int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
            int nprtn, int tpw, hipStream_t st, int* used_kernel)
 {

@@ -158,5 +158,9 @@ int mode_features(const blco_dev* d, int mode, int rank, int ftype, float* out);
 bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int* nprtn);
 int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
            int nprtn, int tpw, hipStream_t st, int* used_kernel);
+// The host-side and device-side set-up that mttkrp() of (mode, rank, ftype,
+// kernel) would do on its first call (kernel 14's row-ordered copy), without
+// a launch; a copy that does not fit leaves the mode to the row window.
+int mttkrp_prepare(blco_dev* d, int mode, int rank, int ftype, int kernel);
 
 }  // namespace blco

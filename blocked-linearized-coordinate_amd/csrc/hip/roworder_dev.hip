@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -52,24 +53,46 @@ uint32_t grid_for(uint64_t n)
     return (uint32_t)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
+// The lowest LDS_ROWS indices are counted in LDS first and added to the
+// global counters once per workgroup: skewed (Zipf) modes put their hot rows
+// at the low indices, whose global atomics would otherwise serialise.
+constexpr uint32_t LDS_ROWS = 8192;
+constexpr uint32_t HIST_GRID = 2048;
+
 __global__ __launch_bounds__(WG) void k_split_hist(uint64_t n, Decode dc, int split, uint64_t mask, int pos,
                                                    unsigned long long* __restrict__ cnt)
 {
+    __shared__ unsigned int h[LDS_ROWS];
+    for (uint32_t t = threadIdx.x; t < LDS_ROWS; t += WG) h[t] = 0;
+    __syncthreads();
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
         const uint64_t b = dc.block(i);
-        atomicAdd(&cnt[dc.index(i, b, mask, pos, split)], 1ull);
+        const uint64_t x = dc.index(i, b, mask, pos, split);
+        if (x < LDS_ROWS) atomicAdd(&h[x], 1u);
+        else atomicAdd(&cnt[x], 1ull);
     }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < LDS_ROWS; t += WG)
+        if (h[t]) atomicAdd(&cnt[t], (unsigned long long)h[t]);
 }
 
 __global__ __launch_bounds__(WG) void k_row_tiles(uint64_t n, Decode dc, int mode, uint64_t rmask, int rpos, int split,
                                                   uint64_t smask, int spos, const uint8_t* __restrict__ tile_of,
                                                   unsigned int* __restrict__ met)
 {
+    __shared__ unsigned int h[LDS_ROWS];
+    for (uint32_t t = threadIdx.x; t < LDS_ROWS; t += WG) h[t] = 0;
+    __syncthreads();
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) {
         const uint64_t b = dc.block(i);
         const uint64_t r = dc.index(i, b, rmask, rpos, mode);
-        atomicOr(&met[r], 1u << tile_of[dc.index(i, b, smask, spos, split)]);
+        const unsigned int bit = 1u << tile_of[dc.index(i, b, smask, spos, split)];
+        if (r < LDS_ROWS) atomicOr(&h[r], bit);
+        else atomicOr(&met[r], bit);
     }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < LDS_ROWS; t += WG)
+        if (h[t]) atomicOr(&met[t], h[t]);
 }
 
 struct KeySpec {
@@ -178,8 +201,8 @@ int dev_split_counts(const blco_dev* d, int split, std::vector<uint64_t>* cnt)
     if ((rc = upload_bstart(d, &B, &bs)) || (rc = B.alloc(&c, I))) return rc;
     HIP_TRY(hipMemset(c, 0, I * 8));
     const Decode dc{d->d_coords, bs, d->d_bcoords, d->nblocks, d->nmodes};
-    hipLaunchKernelGGL(k_split_hist, dim3(grid_for(d->nnz)), dim3(WG), 0, 0, d->nnz, dc, split, d->masks[split],
-                       d->pos[split], c);
+    hipLaunchKernelGGL(k_split_hist, dim3(std::min(grid_for(d->nnz), HIST_GRID)), dim3(WG), 0, 0, d->nnz, dc, split,
+                       d->masks[split], d->pos[split], c);
     HIP_TRY(hipGetLastError());
     cnt->resize(I);
     HIP_TRY(hipMemcpy(cnt->data(), c, I * 8, hipMemcpyDeviceToHost));
@@ -200,8 +223,8 @@ int dev_row_tiles(const blco_dev* d, int mode, int split, const std::vector<uint
     HIP_TRY(hipMemcpy(to, tile_of.data(), tile_of.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(m, 0, rows * 4));
     const Decode dc{d->d_coords, bs, d->d_bcoords, d->nblocks, d->nmodes};
-    hipLaunchKernelGGL(k_row_tiles, dim3(grid_for(d->nnz)), dim3(WG), 0, 0, d->nnz, dc, mode, d->masks[mode],
-                       d->pos[mode], split, d->masks[split], d->pos[split], to, m);
+    hipLaunchKernelGGL(k_row_tiles, dim3(std::min(grid_for(d->nnz), HIST_GRID)), dim3(WG), 0, 0, d->nnz, dc, mode,
+                       d->masks[mode], d->pos[mode], split, d->masks[split], d->pos[split], to, m);
     HIP_TRY(hipGetLastError());
     std::vector<unsigned int> h(rows);
     HIP_TRY(hipMemcpy(h.data(), m, rows * 4, hipMemcpyDeviceToHost));

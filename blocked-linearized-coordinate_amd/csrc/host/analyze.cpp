@@ -12,6 +12,8 @@
 
 namespace blco {
 
+constexpr uint64_t LOCAL_ROWS = 1u << 16;
+
 int analyze_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos, uint64_t nnz,
                    const uint64_t* coords, uint64_t nblocks, const uint64_t* block_start,
                    const uint64_t* block_nnz, const uint64_t* block_coords, uint32_t fine_chunk, uint32_t hot_max,
@@ -37,22 +39,36 @@ int analyze_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks, cons
         uint32_t* c = cnt.data();
         const uint64_t mk = masks[m], dm = dims[m];
         const int ps = mk ? pos[m] : 0;
+        // the lowest LOCAL_ROWS rows (where skewed modes keep their hot rows)
+        // are counted per thread and merged, the rest with atomics
+        const uint64_t local = std::min<uint64_t>(dm, LOCAL_ROWS);
         for (uint64_t b = 0; b < nblocks; ++b) {
             const uint64_t bc = block_coords[b * nmodes + m];
             const uint64_t lo = block_start[b], hi = lo + block_nnz[b];
             uint64_t worst = 0;
             bool out = false;
-            #pragma omp parallel for schedule(static) reduction(max : worst) reduction(|| : out) if (par)
-            for (uint64_t i = lo; i < hi; ++i) {
-                const uint64_t v = ((coords[i] >> ps) & mk) | bc;
-                if (v >= dm) {
-                    out = true;
-                    worst = v > worst ? v : worst;
-                    continue;
+            #pragma omp parallel reduction(max : worst) reduction(|| : out) if (par)
+            {
+                std::vector<uint32_t> mine(par && count ? local : 0, 0);
+                #pragma omp for schedule(static)
+                for (uint64_t i = lo; i < hi; ++i) {
+                    const uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                    if (v >= dm) {
+                        out = true;
+                        worst = v > worst ? v : worst;
+                        continue;
+                    }
+                    if (!count) continue;
+                    if (!par) {
+                        if (c[v] != 0xFFFFFFFFu) ++c[v];
+                    } else if (v < local) {
+                        ++mine[v];
+                    } else {
+                        __atomic_fetch_add(&c[v], 1u, __ATOMIC_RELAXED);
+                    }
                 }
-                if (!count) continue;
-                if (par) __atomic_fetch_add(&c[v], 1u, __ATOMIC_RELAXED);
-                else if (c[v] != 0xFFFFFFFFu) ++c[v];
+                for (uint64_t v = 0; v < mine.size(); ++v)
+                    if (mine[v]) __atomic_fetch_add(&c[v], mine[v], __ATOMIC_RELAXED);
             }
             if (out)
                 return fail(BLCO_EINVAL, "block %llu decodes mode-%d index %llu >= %llu", (unsigned long long)b, m,

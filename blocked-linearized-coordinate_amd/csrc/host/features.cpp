@@ -95,11 +95,23 @@ void compute_static_features(int N, const uint64_t* dims, const uint64_t* masks,
             std::vector<uint32_t> cnt(dims[n], 0);
             uint32_t* cp = cnt.data();
             if (nnz < 0xFFFFFFFFull) {
-                // no count can reach the saturation value: atomic increments in parallel
+                // no count can reach the saturation value: in parallel, the
+                // lowest rows (where skewed modes keep their hot rows) counted
+                // per thread and merged, the rest with atomic increments
+                const uint64_t local = std::min<uint64_t>(dims[n], 1u << 16);
                 for (uint64_t b = 0; b < nblocks; ++b) {
-                    #pragma omp parallel for schedule(static)
-                    for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i)
-                        __atomic_fetch_add(&cp[index(b, i, n)], 1u, __ATOMIC_RELAXED);
+                    #pragma omp parallel
+                    {
+                        std::vector<uint32_t> mine(local, 0);
+                        #pragma omp for schedule(static)
+                        for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
+                            const uint64_t v = index(b, i, n);
+                            if (v < local) ++mine[v];
+                            else __atomic_fetch_add(&cp[v], 1u, __ATOMIC_RELAXED);
+                        }
+                        for (uint64_t v = 0; v < local; ++v)
+                            if (mine[v]) __atomic_fetch_add(&cp[v], mine[v], __ATOMIC_RELAXED);
+                    }
                 }
             } else {
                 for (uint64_t b = 0; b < nblocks; ++b)
