@@ -99,6 +99,9 @@ def lib():
             "blco_dev_create_arrays": (C.c_int, [C.c_int, u64p, u64p, i32p, C.c_uint64, u64p, f64p,
                                                  C.c_uint64, u64p, u64p, u64p, C.c_int, C.c_int,
                                                  C.c_uint64, C.POINTER(vp)]),
+            "blco_dev_create_budget": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_uint64, C.POINTER(vp)]),
+            "blco_dev_residency": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64)]),
             "blco_dev_free": (None, [vp]),
             "blco_generate_device": (C.c_int, [C.POINTER(GenSpec), C.c_uint64, C.c_int, C.POINTER(vp)]),
             "blco_dev_to_host": (C.c_int, [vp, C.POINTER(vp)]),
@@ -449,10 +452,17 @@ def _gen_spec(dims, nnz, zipf, gen_rank, value_kind, seed, topup):
 class DeviceTensor:
     """BLCO tensor in HBM (or streamed), gen_blcotensor_device."""
 
-    def __init__(self, host, value_dtype=F64, stream_data=False, slot_nnz=0):
+    def __init__(self, host, value_dtype=F64, stream_data=False, slot_nnz=0, device_budget=0):
+        """device_budget > 0: blco_dev_create_budget (resident when the
+        nonzeros fit, else a resident prefix plus streaming through slots of
+        slot_nnz nonzeros; stream_data is then ignored)."""
         h = vp()
-        _check(lib().blco_dev_create(host.h, value_dtype, 1 if stream_data else 0, slot_nnz,
-                                     C.byref(h)))
+        if device_budget:
+            _check(lib().blco_dev_create_budget(host.h, value_dtype, int(device_budget), slot_nnz,
+                                                C.byref(h)))
+        else:
+            _check(lib().blco_dev_create(host.h, value_dtype, 1 if stream_data else 0, slot_nnz,
+                                         C.byref(h)))
         self.h = h.value
         self.nmodes = host.nmodes
         self.dims = host.dims.copy()
@@ -476,6 +486,12 @@ class DeviceTensor:
         self.nmodes, self.dims, self.nnz, self.nblocks = n.value, dims[:n.value].copy(), nnz.value, nb.value
         self.value_dtype = value_dtype
         return self
+
+    def residency(self):
+        """(resident nonzeros, streamed nonzeros, device bytes)."""
+        r, s_, b = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        _check(lib().blco_dev_residency(self.h, C.byref(r), C.byref(s_), C.byref(b)))
+        return r.value, s_.value, b.value
 
     def to_host(self):
         """Host copy (BlcoHost, values as f64) of a resident tensor."""

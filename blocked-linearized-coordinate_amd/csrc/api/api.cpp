@@ -300,11 +300,27 @@ blcotensor* gen_blcotensor_host_any(SparseTensor* X, IType max_block_size)
     return B;
 }
 
+static uint64_t g_budget = 0, g_budget_slot = 0;
+void cpals_set_device_budget(uint64_t bytes, uint64_t slot_nnz)
+{
+    g_budget = bytes;
+    g_budget_slot = slot_nnz;
+}
+
+// The device tensor of the drivers: under a device budget the budget decides
+// what is resident (slots of the streaming block size when streaming was
+// asked for, else of the budget's own slot size).
+static int create_dev(const blco_host* h, int value_dtype, bool stream_data, uint64_t slot_nnz, blco_dev** out)
+{
+    if (g_budget) return blco_dev_create_budget(h, value_dtype, g_budget, stream_data ? slot_nnz : g_budget_slot, out);
+    return blco_dev_create(h, value_dtype, stream_data ? 1 : 0, stream_data ? slot_nnz : 0, out);
+}
+
 blcotensor* gen_blcotensor_device(blcotensor* X, IType /*block_count*/, IType MB,
                                   bool /*do_batching*/, int value_dtype)
 {
     blcotensor* D = new blcotensor;
-    die_if(blco_dev_create(X->host, value_dtype, MB > 0 ? 1 : 0, MB, &D->dev), "gen_blcotensor_device");
+    die_if(create_dev(X->host, value_dtype, MB > 0, MB, &D->dev), "gen_blcotensor_device");
     D->N = X->N;
     D->modes = new IType[X->N];
     std::memcpy(D->modes, X->modes, sizeof(IType) * X->N);
@@ -465,8 +481,7 @@ FType cpals_blco_dev(blcotensor* X, KruskalModel* A, int maxiter, FType toleranc
     printf("\nCopying to GPU device\n--> Init BLCO on GPU\n");
     double t0 = omp_get_wtime();
     blco_dev* d = nullptr;
-    die_if(blco_dev_create(X->host, g_precision == 0 ? BLCO_F64 : BLCO_F32, stream_data ? 1 : 0,
-                           stream_data ? X->max_block_size : 0, &d),
+    die_if(create_dev(X->host, g_precision == 0 ? BLCO_F64 : BLCO_F32, stream_data, X->max_block_size, &d),
            "gen_blcotensor_device");
     printf("--> Total initialization = %0.3f (s)\n\n", omp_get_wtime() - t0);
     blco_cpals_opts o;

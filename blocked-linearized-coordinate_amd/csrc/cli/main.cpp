@@ -3,7 +3,8 @@
 //   "--> Iter %3d: f = ... f-delta = ... time = ..., (s)", "Final fit: %0.9f",
 //   "Total time for MTTKRP: %0.3f (time per iter: %0.3f ms)",
 //   "Results of <name> is correct" / "... is incorrect by <count>".
-// Additions: --precision f64|f32|bf16, --mode-order natural|longest, --graph;
+// Additions: --precision f64|f32|bf16, --mode-order natural|longest, --graph,
+// --device-budget BYTES;
 // --max_block_size is accepted beside --max-block-size.  Without -k the
 // reference runs its CPU ALTO path; this build runs the GPU path with the
 // AUTO kernel (-k 10) and says so.
@@ -38,6 +39,7 @@ static const struct option long_opt[] = {
     {"max-block-size", 1, NULL, 1004}, {"max_block_size", 1, NULL, 1004},
     {"batch", 0, NULL, 1005},        {"precision", 1, NULL, 1006},
     {"mode-order", 1, NULL, 1007},   {"graph", 0, NULL, 1008},
+    {"device-budget", 1, NULL, 1009},
     {NULL, 0, NULL, 0}};
 static const char* short_opt = "hvi:o:b:r:m:x:d:t:s:e:f:cpk:n:";
 
@@ -71,7 +73,25 @@ static void Usage(const char* call)
             "\t--batch              Launch all blocks at once\n"
             "\t--precision          f64 (default), f32 or bf16 factors\n"
             "\t--mode-order         natural (default) or longest\n"
-            "\t--graph              Replay ALS sweeps from a captured hipGraph\n");
+            "\t--graph              Replay ALS sweeps from a captured hipGraph\n"
+            "\t--device-budget      Device bytes for the tensor (suffix K, M or G: KiB, MiB,\n"
+            "\t                     GiB); what does not fit streams in blocks of\n"
+            "\t                     --max-block-size nonzeros\n");
+}
+
+// "32G", "512M", "4096": bytes, with binary K / M / G suffixes; 0 on error.
+static uint64_t ParseBytes(const char* arg)
+{
+    char* end = nullptr;
+    const double v = strtod(arg, &end);
+    if (!end || end == arg || v <= 0.0) return 0;
+    double mul = 1.0;
+    if (*end == 'K' || *end == 'k') mul = 1024.0, ++end;
+    else if (*end == 'M' || *end == 'm') mul = 1048576.0, ++end;
+    else if (*end == 'G' || *end == 'g') mul = 1073741824.0, ++end;
+    if (*end == 'B' || *end == 'b') ++end;
+    if (*end) return 0;
+    return (uint64_t)(v * mul);
 }
 
 static std::vector<IType> ParseDimensions(const char* arg)
@@ -143,6 +163,7 @@ int main(int argc, char** argv)
     bool do_check = false, do_bench = false, stream_data = false, do_batching = false;
     int kernel_id = 0, num_partitions = 0, device = 0, precision = 0, mode_order = 0, graph = 0;
     IType thread_cf = 2, max_block_size = 16777216;
+    uint64_t device_budget = 0;
 
     int c;
     while ((c = getopt_long(argc, argv, short_opt, long_opt, NULL)) != -1) {
@@ -177,6 +198,7 @@ int main(int argc, char** argv)
                 break;
             case 1007: mode_order = !strcmp(optarg, "longest") ? 1 : 0; break;
             case 1008: graph = 1; break;
+            case 1009: device_budget = ParseBytes(optarg); if (!device_budget) { fprintf(stderr, "Invalid --device-budget: %s.\n", optarg); return -1; } break;
             default: Usage(argv[0]); return -1;
         }
     }
@@ -223,6 +245,7 @@ int main(int argc, char** argv)
     }
     if (hipSetDevice(device) != hipSuccess) { fprintf(stderr, "hipSetDevice failed\n"); return -1; }
     if (do_check) max_iters = 1;
+    cpals_set_device_budget(device_budget, max_block_size);
     if (!stream_data) max_block_size = X->nnz;
     if (num_partitions == 0) num_partitions = omp_get_max_threads();  // src/main.cpp:125
     cpals_set_precision(precision);

@@ -449,6 +449,19 @@ static int xcd_tiles(const blco_dev* d, int mode, int split, uint64_t row_bytes,
     return added * XCD_TILE_RUN_RATIO <= n ? NUM_XCD : 1;
 }
 
+// Device bytes that hold nonzeros: the resident ones, the two streaming
+// slots and the row-ordered copies (what a device budget caps; the chunk,
+// window and hot-row tables are outside it).
+static uint64_t nonzero_bytes(const blco_dev* d)
+{
+    const uint64_t per_nz = 8 + (uint64_t)value_size(d->vdtype);
+    uint64_t b = d->res_nnz * per_nz;
+    if (d->stream_data) b += 2 * std::min<uint64_t>(d->slot_nnz, d->nnz ? d->nnz : 1) * per_nz;
+    for (int m = 0; m < d->nmodes; ++m)
+        if (d->d_scoords[m]) b += d->nnz * per_nz;
+    return b;
+}
+
 // Builds the row-ordered copy of a mode (kernel 14) from the resident
 // tensor, once per mode: on the GPU (roworder_dev.hip), or with
 // BLCO_ROWORDER=host by a download, a stable sort on the host and an upload
@@ -463,6 +476,10 @@ static int ensure_row_ordered(blco_dev* d, int mode, int rank, int ftype)
     const uint64_t n = d->nnz, nb = d->nblocks;
     const int N = d->nmodes;
     const size_t vsz = value_size(d->vdtype);
+    if (d->budget && nonzero_bytes(d) + n * (8 + vsz) > d->budget)
+        return fail(BLCO_ENOMEM, "the row-ordered copy of mode %d (%llu bytes) exceeds the device budget "
+                    "(%llu of %llu bytes held by nonzeros)", mode, (unsigned long long)(n * (8 + vsz)),
+                    (unsigned long long)nonzero_bytes(d), (unsigned long long)d->budget);
     const char* renv = getenv("BLCO_ROWORDER");
     bool on_host = renv && std::strcmp(renv, "host") == 0;
     progress("row-ordered copy of mode %d: %llu nonzeros (%s)", mode, (unsigned long long)n,
@@ -883,8 +900,18 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
                     if (cnt > d->slot_nnz)
                         return fail(BLCO_EUNSUPPORTED, "a window of %llu nonzeros exceeds the %llu-nonzero slot",
                                     (unsigned long long)cnt, (unsigned long long)d->slot_nnz);
-                    const int s = (int)(u & 1);
                     const uint64_t begin = wt->h_sc[e].begin;
+                    a.chunk0 = e;
+                    a.nchunks = e1 - e;
+                    if (begin + cnt <= d->res_nnz) {  // within the resident prefix
+                        a.coords = d->d_coords;
+                        a.vals = d->d_vals;
+                        a.coord_base = 0;
+                        rc = launch_mttkrp(wk, a, ftype, a.nchunks, lds, st);
+                        e = e1;
+                        continue;
+                    }
+                    const int s = (int)(u & 1);
                     HIP_TRY(hipStreamWaitEvent(d->copy_stream, d->ev_done[s], 0));
                     HIP_TRY(hipMemcpyAsync(d->slot_coords[s], d->h_coords + begin, cnt * 8,
                                            hipMemcpyHostToDevice, d->copy_stream));
@@ -895,8 +922,6 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
                     a.coords = d->slot_coords[s];
                     a.vals = d->slot_vals[s];
                     a.coord_base = begin;
-                    a.chunk0 = e;
-                    a.nchunks = e1 - e;
                     rc = launch_mttkrp(wk, a, ftype, a.nchunks, lds, st);
                     HIP_TRY(hipEventRecord(d->ev_done[s], st));
                     ++u;
@@ -936,11 +961,20 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
             rc = launch_range(d, which, a, t, 0, t->n, ftype, st);
         }
     } else {
-        // double-buffered streaming: unit u goes through slot u % 2
+        // double-buffered streaming: the k-th unit beyond the resident
+        // prefix goes through slot k % 2
         const int vsz = value_size(d->vdtype);
+        size_t k = 0;
         for (size_t u = 0; u < t->units.size() && !rc; ++u) {
             const StreamUnit& su = t->units[u];
-            const int s = (int)(u & 1);
+            if (su.begin + su.nnz <= d->res_nnz) {
+                a.coords = d->d_coords;
+                a.vals = d->d_vals;
+                a.coord_base = 0;
+                rc = launch_range(d, which, a, t, su.c0, su.c1, ftype, st);
+                continue;
+            }
+            const int s = (int)(k++ & 1);
             HIP_TRY(hipStreamWaitEvent(d->copy_stream, d->ev_done[s], 0));
             HIP_TRY(hipMemcpyAsync(d->slot_coords[s], d->h_coords + su.begin, su.nnz * 8,
                                    hipMemcpyHostToDevice, d->copy_stream));
@@ -1053,7 +1087,8 @@ int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks
                             uint64_t nnz, const uint64_t* coords, const double* vals,
                             uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz,
                             const uint64_t* block_coords, int value_dtype, int stream_data,
-                            uint64_t slot_nnz, uint64_t* adopt_coords, void* adopt_vals, blco_dev** out)
+                            uint64_t slot_nnz, uint64_t device_budget, uint64_t* adopt_coords, void* adopt_vals,
+                            blco_dev** out)
 {
     const bool adopt = adopt_coords != nullptr;
     auto drop_adopted = [&](int rc) {
@@ -1100,9 +1135,24 @@ int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks
     std::vector<std::vector<uint32_t>>& hot = tt.hot;
     std::vector<std::vector<uint64_t>>& hot_cnt = tt.hot_cnt;
     if (!have_kfd()) return drop_adopted(fail(BLCO_ENODEV, "no GPU (/dev/kfd absent)"));
+    const int vsz = value_size(value_dtype);
+    const uint64_t per_nz = 8 + (uint64_t)vsz;
+    // a device budget keeps a tensor that fits resident and streams the
+    // rest of one that does not; the nonzeros it cannot hold beside the two
+    // slots pass through them
+    uint64_t res_nnz = 0;
+    if (device_budget && !adopt) stream_data = nnz * per_nz > device_budget;
     if (stream_data && slot_nnz == 0) slot_nnz = 1ull << 24;  // --max-block-size default
     // a slot holds at least one chunk at the largest tiles-per-wave (64 x 64)
     if (stream_data && slot_nnz < 4096) slot_nnz = 4096;
+    if (stream_data && device_budget) {
+        const uint64_t slots = 2 * std::min<uint64_t>(slot_nnz, nnz ? nnz : 1) * per_nz;
+        if (device_budget < slots)
+            return drop_adopted(fail(BLCO_EINVAL, "device budget of %llu bytes is below the two streaming slots "
+                                     "(%llu bytes at %llu nonzeros each)", (unsigned long long)device_budget,
+                                     (unsigned long long)slots, (unsigned long long)slot_nnz));
+        res_nnz = std::min<uint64_t>(nnz, (device_budget - slots) / per_nz);
+    }
 
     blco_dev* d = new (std::nothrow) blco_dev();
     if (!d) return drop_adopted(fail(BLCO_ENOMEM, "out of host memory"));
@@ -1119,6 +1169,8 @@ int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks
     d->vdtype = value_dtype;
     d->stream_data = stream_data ? 1 : 0;
     d->slot_nnz = slot_nnz;
+    d->res_nnz = stream_data ? res_nnz : nnz;
+    d->budget = device_budget;
     d->nblocks = nblocks;
     d->bstart.assign(block_start, block_start + nblocks);
     d->bnnz.assign(block_nnz, block_nnz + nblocks);
@@ -1159,7 +1211,6 @@ int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks
             return cleanup(fail(BLCO_EHIP, "upload of block coordinates failed"));
         d->bytes += sizeof(uint64_t) * nblocks * nmodes;
     }
-    const int vsz = value_size(value_dtype);
     // values converted to the device dtype through a bounded host staging buffer
     auto convert = [&](uint64_t off, uint64_t cnt, void* dst) {
         if (value_dtype == BLCO_F64) std::memcpy(dst, vals + off, cnt * 8);
@@ -1197,6 +1248,15 @@ int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks
             std::memcpy(d->h_coords, coords, nnz * 8);
             convert(0, nnz, d->h_vals);
         }
+        if (res_nnz) {  // the resident prefix, from the pinned copy
+            if (hipMalloc(&d->d_coords, res_nnz * 8) != hipSuccess || hipMalloc(&d->d_vals, res_nnz * vsz) != hipSuccess)
+                return cleanup(fail(BLCO_ENOMEM, "device allocation of %llu resident nonzeros failed",
+                                    (unsigned long long)res_nnz));
+            d->bytes += res_nnz * per_nz;
+            if (hipMemcpy(d->d_coords, d->h_coords, res_nnz * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d->d_vals, d->h_vals, res_nnz * vsz, hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(fail(BLCO_EHIP, "upload of the resident nonzeros failed"));
+        }
         uint64_t cap = std::min<uint64_t>(slot_nnz, nnz ? nnz : 1);
         for (int s = 0; s < 2; ++s) {
             if (hipMalloc(&d->slot_coords[s], cap * 8) != hipSuccess || hipMalloc(&d->slot_vals[s], cap * vsz) != hipSuccess)
@@ -1225,7 +1285,26 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims, const uint64_t* mas
                            uint64_t slot_nnz, blco_dev** out)
 {
     return create_device_tensor(nmodes, dims, masks, pos, nnz, coords, vals, nblocks, block_start, block_nnz,
-                                block_coords, value_dtype, stream_data, slot_nnz, nullptr, nullptr, out);
+                                block_coords, value_dtype, stream_data, slot_nnz, 0, nullptr, nullptr, out);
+}
+
+int blco_dev_create_budget(const blco_host* h, int value_dtype, uint64_t device_budget, uint64_t slot_nnz,
+                           blco_dev** out)
+{
+    if (!h) return fail(BLCO_EINVAL, "null host tensor");
+    return create_device_tensor(h->L.nmodes, h->L.dims, h->L.masks, h->L.pos, h->nnz, h->coords.data(),
+                                h->vals.data(), h->block_nnz.size(), h->block_start.data(), h->block_nnz.data(),
+                                h->block_coords.data(), value_dtype, 0, slot_nnz, device_budget, nullptr, nullptr,
+                                out);
+}
+
+int blco_dev_residency(const blco_dev* d, uint64_t* resident_nnz, uint64_t* streamed_nnz, uint64_t* device_bytes)
+{
+    if (!d) return fail(BLCO_EINVAL, "null tensor");
+    if (resident_nnz) *resident_nnz = d->res_nnz;
+    if (streamed_nnz) *streamed_nnz = d->nnz - d->res_nnz;
+    if (device_bytes) *device_bytes = d->bytes;
+    return BLCO_OK;
 }
 
 int blco_dev_create(const blco_host* h, int value_dtype, int stream_data, uint64_t slot_nnz,

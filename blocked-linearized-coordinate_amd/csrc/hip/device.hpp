@@ -120,7 +120,10 @@ struct blco_dev {
     // selector features (rank- and type-independent entries) and decisions
     float sfeat[BLCO_MAX_MODES][BLCO_SEL_NFEAT] = {};
     std::vector<blco::SelCache> sel_cache;
-    // streaming
+    // streaming: the whole tensor in pinned host memory; its first res_nnz
+    // nonzeros also stay on the device (the rest pass through the slots)
+    uint64_t res_nnz = 0;
+    uint64_t budget = 0;  // device bytes for nonzeros, slots and copies (0: no cap)
     uint64_t* h_coords = nullptr;
     void* h_vals = nullptr;
     uint64_t* slot_coords[2] = {nullptr, nullptr};

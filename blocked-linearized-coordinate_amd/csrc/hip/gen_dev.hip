@@ -45,7 +45,8 @@ int create_device_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks
                          uint64_t nnz, const uint64_t* coords, const double* vals,
                          uint64_t nblocks, const uint64_t* block_start, const uint64_t* block_nnz,
                          const uint64_t* block_coords, int value_dtype, int stream_data,
-                         uint64_t slot_nnz, uint64_t* adopt_coords, void* adopt_vals, blco_dev** out);
+                         uint64_t slot_nnz, uint64_t device_budget, uint64_t* adopt_coords, void* adopt_vals,
+                         blco_dev** out);
 
 namespace {
 
@@ -557,6 +558,6 @@ extern "C" int blco_generate_device(const blco_gen_spec* sp, uint64_t max_block_
         if (q == lo || q == dvals) q = nullptr;
     progress("GPU generator: encoded, %zu block(s)", h.block_start.size());
     return create_device_tensor(N, L.dims, L.masks, L.pos, nnz, hc.data(), nullptr, h.block_start.size(),
-                                h.block_start.data(), h.block_nnz.data(), h.block_coords.data(), value_dtype, 0, 0,
+                                h.block_start.data(), h.block_nnz.data(), h.block_coords.data(), value_dtype, 0, 0, 0,
                                 nnz ? lo : nullptr, nnz ? dvals : nullptr, out);
 }

@@ -14,6 +14,11 @@ void cpals_set_precision(int precision);
 void cpals_set_mode_order(int order);
 // Capture the ALS sweep in a hipGraph and replay it (1) or launch eagerly (0).
 void cpals_set_graph(int use_graph);
+// Device budget (bytes) for the nonzeros of every device tensor that
+// cpals_blco_dev and the MTTKRP drivers create: a tensor that fits stays
+// resident, a larger one keeps what fits resident and streams the rest
+// through two slots of slot_nnz nonzeros (0: no budget).
+void cpals_set_device_budget(uint64_t bytes, uint64_t slot_nnz);
 
 FType cpals_blco_dev(blcotensor* X, KruskalModel* A, int maxiter, FType tolerance, int kernel,
                      bool stream_data, bool do_batching, int thread_cf, int nprtn);

@@ -93,6 +93,25 @@ int blco_dev_create_arrays(int nmodes, const uint64_t* dims,
                            const uint64_t* block_nnz,
                            const uint64_t* block_coords, int value_dtype,
                            int stream_data, uint64_t slot_nnz, blco_dev** out);
+/* gen_blcotensor_device under a device-memory budget (no counterpart
+ * flag in the reference, whose --stream-data streams every block through
+ * NUM_STREAMS slots, src/alto_dev.cu:1869-1874, 1987).  device_budget caps
+ * the device bytes that hold nonzeros: a tensor whose nonzeros fit
+ * (nnz * (8 + value size) <= device_budget) is resident as with
+ * blco_dev_create(h, value_dtype, 0, 0, out); a larger one keeps the whole
+ * tensor in pinned host memory, keeps its leading nonzeros (ALTO order) on
+ * the device as far as the budget allows beside two slots of slot_nnz
+ * nonzeros (0: 2^24), and streams the rest through those slots.  The
+ * budget also caps kernel 14's row-ordered copies (each nnz * (8 + value
+ * size) bytes): a copy that would exceed it is not built and the mode runs
+ * the row window instead.  BLCO_EINVAL when the two slots alone exceed
+ * the budget. */
+int blco_dev_create_budget(const blco_host* h, int value_dtype, uint64_t device_budget,
+                           uint64_t slot_nnz, blco_dev** out);
+/* Nonzeros held on the device and streamed from host memory per MTTKRP,
+ * and the device bytes the tensor holds (blco_dev_bytes). */
+int blco_dev_residency(const blco_dev* d, uint64_t* resident_nnz, uint64_t* streamed_nnz,
+                       uint64_t* device_bytes);
 /* delete_blcotensor_device, src/blco.cu:74-95 */
 void blco_dev_free(blco_dev* d);
 /* Bytes of device memory held by the tensor. */

@@ -34,6 +34,29 @@ def test_without_gpu_fails_cleanly(B, tmp_path):
     assert "GPU" in r.stderr
 
 
+def test_device_budget_flag_parsing():
+    r = run("--device-budget", "12Q", "-d", "10,10,10")
+    assert r.returncode != 0 and "Invalid --device-budget" in r.stderr
+    assert "--device-budget" in run("-h").stderr
+
+
+@pytest.mark.gpu
+def test_device_budget_flag_matches_resident(B, gpu, tmp_path):
+    """--device-budget 256K holds 8192 of the 20000 nonzeros beside two
+    4096-nonzero slots (16 bytes each); the run's fits equal the resident
+    run's."""
+    t = B.Coo.generate((300, 200, 100), 20000, gen_rank=5, value_kind=1, seed=9)
+    path = str(tmp_path / "t.bin")
+    t.save(path, 1)
+    common = ("-b", path, "-r", "8", "-m", "4", "-x", "5", "-e", "0", "-k", "10")
+    r0 = run(*common)
+    r1 = run(*common, "--device-budget", "256K", "--max-block-size", "4096")
+    assert r0.returncode == 0 and r1.returncode == 0, (r0.stderr, r1.stderr)
+    f0 = float(re.search(r"Final fit: ([0-9.eE+-]+)", r0.stdout).group(1))
+    f1 = float(re.search(r"Final fit: ([0-9.eE+-]+)", r1.stdout).group(1))
+    assert abs(f0 - f1) <= 1e-9, (f0, f1)
+
+
 @pytest.mark.gpu
 def test_cpals_fit_matches_oracle(B, O, gpu, tmp_path):
     dims, R, iters, seed = (300, 200, 100), 8, 6, 42

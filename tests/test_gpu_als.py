@@ -318,6 +318,19 @@ def test_cpals_streamed_matches_resident(B, gpu, small):
     assert np.abs(fa - fb).max() <= 1e-10
 
 
+def test_cpals_device_budget_matches_resident(B, gpu, small):
+    """CP-ALS on a tensor of which a device budget holds about a third
+    (the rest streamed) follows the resident trajectory to 1e-10."""
+    dims, t, h, at = small
+    U0 = B.init_factors(4, dims, 16)
+    a = B.DeviceTensor(h)
+    b = B.DeviceTensor(h, slot_nnz=20000, device_budget=(2 * 20000 + h.nnz // 3) * 16)
+    assert 0 < b.residency()[1] < h.nnz
+    _, _, fa = B.cpals(a, U0, 16, 5, 0.0)
+    _, _, fb = B.cpals(b, U0, 16, 5, 0.0)
+    assert np.abs(fa - fb).max() <= 1e-10
+
+
 def test_cpals_mode_order_longest_first(B, O, gpu, small):
     dims, t, h, at = small
     U0 = B.init_factors(6, dims, 8)

@@ -179,6 +179,57 @@ def test_streamed_blocks(B, O, gpu, tensors, kernel):
     check(B, O, tensors, "bits65", 1, getattr(B, kernel), 32, stream=True)
 
 
+@pytest.mark.parametrize("kernel", ["K_LVL1", "K_BATCHED", "K_BATCHED2", "K_ATOMIC", "K_LVL3", "K_AUTO"])
+def test_device_budget_streams_the_rest(B, O, gpu, tensors, kernel):
+    """A device budget below the tensor's size keeps the leading half of the
+    nonzeros resident and streams the other half through two slots; every
+    mode's MTTKRP matches the oracle, and (FP64) the resident tensor's to
+    1e-10."""
+    for name, dtype, rank in (("manyblocks", 0, 16), ("bits65", 1, 32)):
+        t, h, at = tensors[name]
+        vdt = 0 if dtype == 0 else 1
+        per = 8 + (8 if dtype == 0 else 4)
+        slot = 4096  # two slots hold less than a quarter of either tensor
+        assert 4 * slot < h.nnz
+        budget = 2 * slot * per + (h.nnz // 2) * per
+        dev = B.DeviceTensor(h, value_dtype=vdt, slot_nnz=slot, device_budget=budget)
+        full = B.DeviceTensor(h, value_dtype=vdt)
+        res, streamed, _ = dev.residency()
+        assert (res, streamed) == (h.nnz // 2, h.nnz - h.nnz // 2)
+        assert full.residency()[:2] == (h.nnz, 0)
+        rng = np.random.default_rng(rank)
+        U = [rng.random((int(d), rank)) for d in t.dims]
+        print("case", name, "budget", budget, "kernel", kernel, flush=True)
+        for m in range(t.nmodes):
+            got = run_mttkrp(B, dev, U, m, dtype, getattr(B, kernel))
+            assert rel(got, at.mttkrp(U, m)) <= TOL[dtype], (name, kernel, m)
+            if dtype == 0:
+                ref = run_mttkrp(B, full, U, m, dtype, getattr(B, kernel))
+                assert rel(got, ref) <= 1e-10, (name, kernel, m)
+        dev.free()
+        full.free()
+
+
+def test_device_budget_limits(B, O, gpu, tensors):
+    """A budget that holds the nonzeros keeps the tensor resident but no
+    row-ordered copy beside it (the automatic kernel then runs the row
+    window, still matching the oracle); a budget below the two slots is an
+    error."""
+    t, h, at = tensors["c2shape"]
+    exact = h.nnz * 16
+    dev = B.DeviceTensor(h, device_budget=exact)
+    res, streamed, b0 = dev.residency()
+    assert (res, streamed) == (h.nnz, 0)
+    U = [np.random.default_rng(3).random((int(d), 32)) for d in t.dims]
+    for m in range(t.nmodes):
+        got = run_mttkrp(B, dev, U, m, 0, B.K_AUTO)
+        assert rel(got, at.mttkrp(U, m)) <= TOL[0]
+    assert dev.residency()[2] < b0 + h.nnz * 16  # no copy was built
+    dev.free()
+    with pytest.raises(B.BlcoError):
+        B.DeviceTensor(h, slot_nnz=1 << 16, device_budget=(1 << 16) * 16)
+
+
 def test_accumulates_into_output(B, O, gpu, tensors):
     """The output is accumulated into (the caller zeroes it)."""
     t, h, at = tensors["c1shape"]
