@@ -940,10 +940,10 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
         const ChunkTab* stab = sorted_chunk_table(d, mode, tpw);
         if (!stab) return BLCO_ENOMEM;
         // 32-bit row offsets (and byte offsets into the factors' buffer
-        // resources) when every factor holds fewer than 2^32 bytes
+        // resources) when every factor holds fewer than FACTOR_BYTES_MAX bytes
         bool wide = false;
         for (int m = 0; m < d->nmodes; ++m)
-            wide |= d->dims[m] * (uint64_t)rank * dtype_size(ftype) > 0xFFFFFFFFull;
+            wide |= d->dims[m] * (uint64_t)rank * dtype_size(ftype) >= FACTOR_BYTES_MAX;
         int kind = wide ? W_SORTED64 : W_SORTED;
         if (d->sfiber[mode]) {
             kind = wide ? W_FIBER64 : W_FIBER;

@@ -61,6 +61,9 @@ struct MttkrpArgs {
 enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4, W_SORTED = 5, W_SORTED64 = 6,
        W_FIBER = 7, W_FIBER64 = 8 };
 
+// The row-ordered and fiber kernels take 32-bit byte offsets into factors
+// of fewer bytes than this (a row's offset plus its column bytes then fits).
+constexpr uint32_t FACTOR_BYTES_MAX = 0xFFFFFF00u;
 constexpr uint32_t FINE_CHUNK = 1024;        // nonzeros per range-table entry
 constexpr uint32_t WINDOW_LDS_BYTES = 40 * 1024;  // hot area + row window per workgroup
 constexpr int WIN_WAVES = 8;                      // waves per row-window workgroup

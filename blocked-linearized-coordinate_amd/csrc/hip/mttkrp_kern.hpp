@@ -812,6 +812,246 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
     }
 }
 
+// MTTKRP over a row-ordered copy with every round's decoded nonzeros staged
+// in LDS (the form of kernel 14 for factors of fewer than FACTOR_BYTES_MAX
+// bytes; k_mttkrp_sorted and k_mttkrp_fiber remain for larger ones).  Slots
+// and rounds are those of k_mttkrp_sorted: each lane decodes the nonzero of
+// its own position once and writes one record -- the output row (with the
+// fiber-start bit when FIB), the 32-bit byte offsets of the input factor
+// rows, the value -- to its wave's LDS area; the slot's lanes then read
+// each position's record with LDS loads at immediate offsets, in place of a
+// cross-lane permute per field.  Positions past the end of a slot's run read
+// the run's last nonzero with value 0 (and no fiber start), so the walk has
+// no masked branches.
+//   FIB = false: every input factor row is gathered per nonzero (the plain
+//   row-ordered kernel); the input modes in increasing order.
+//   FIB = true: the copy is in fiber order (k_mttkrp_fiber): the inner row
+//   is gathered per nonzero, the outer rows by the lanes whose position
+//   starts a fiber, and their product scales the fiber's terms.
+// The flush and the final segmented scan over the slots are those of
+// k_mttkrp_sorted.
+template <int N, typename FT, typename VT, int C, int L, bool EXACT, bool FIB>
+__global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
+{
+    using AT = typename Acc<FT>::type;
+    constexpr int S = WAVE / L;
+    constexpr int RW = N + (int)(sizeof(AT) / 4);  // row, N-1 offsets, value
+    constexpr int RWP = RW <= 4 ? 4 : 8;
+    // positions whose gathers are issued together (FP64 rows need twice the
+    // registers: two positions keep 4 waves per SIMD)
+    constexpr int PB = sizeof(AT) == 8 ? 2 : 4;
+    static_assert(L % PB == 0, "positions per batch divide the slot width");
+    // a slot's records, then 16 bytes of padding: the slots' records of one
+    // position then start in different LDS banks
+    constexpr int SLOT_W = L * RWP + 4;
+    __shared__ __attribute__((aligned(16))) uint32_t s_rec[WAVES_PER_WG][S * SLOT_W];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = a.chunk0 + xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
+    if (c >= a.chunk0 + a.nchunks) return;  // whole wave leaves; no block barriers below
+    const Chunk ch = a.chunks[c];
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+    const int slot = lane / L, sub = lane % L;
+    const int R = a.rank, mode = a.mode;
+    // input modes: FIB, the inner one first, then the outer ones
+    int in[N - 1] = {};
+    {
+        int j = 0;
+        if (FIB) in[j++] = a.inner;
+        #pragma unroll
+        for (int m = 0; m < N; ++m)
+            if (m != mode && !(FIB && m == a.inner) && j < N - 1) in[j++] = m;
+    }
+    const uint32_t run = (ch.len + S - 1) / S;
+    const uint32_t p0 = min((uint32_t)slot * run, ch.len), p1 = min(p0 + run, ch.len);
+    const uint32_t rounds = (run + L - 1) / L;  // the same for every slot
+    AT* out = reinterpret_cast<AT*>(a.out);
+    const VT* vals = reinterpret_cast<const VT*>(a.vals);
+    __amdgpu_buffer_rsrc_t rs[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) rs[m] = factor_rsrc(a.f[m]);
+    uint32_t* my_rec = s_rec[w] + slot * SLOT_W + sub * RWP;
+    const uint32_t* slot_rec = s_rec[w] + slot * SLOT_W;
+    for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
+        const int col = cb + sub * C;
+        const uint32_t colb = (uint32_t)col * (uint32_t)sizeof(FT);
+        AT acc[C], scale[C];
+        #pragma unroll
+        for (int cc = 0; cc < C; ++cc) acc[cc] = scale[cc] = 0;
+        uint32_t cur = 0;
+        bool have = false;
+        auto flush = [&]() {
+            AT* o = out + (uint64_t)cur * R + col;
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc)
+                if (EXACT || col + cc < R) atomic_add_global(o + cc, acc[cc]);
+        };
+        auto row_piece = [&](int m, uint32_t byte_off, AT (&x)[C]) {
+            if (EXACT || col + C <= R) {
+                load_row_buf<FT, C>(rs[m], byte_off + colb, x);
+            } else {
+                const FT* f = reinterpret_cast<const FT*>(reinterpret_cast<const char*>(a.f[m]) + byte_off) + col;
+                #pragma unroll
+                for (int cc = 0; cc < C; ++cc) x[cc] = (col + cc < R) ? (AT)to_acc(f[cc]) : (AT)0;
+            }
+        };
+        // a position past the run reads the run's last nonzero (p1 >= 1)
+        auto fetch = [&](uint32_t r, uint64_t& coord, AT& v, bool& valid) {
+            const uint32_t q = p0 + r * L + sub;
+            valid = r < rounds && q < p1;
+            const uint64_t el = ch.begin + (valid ? q : p1 - 1) - a.coord_base;
+            coord = stream_load(a.coords + el);
+            const AT x = (AT)stream_load(vals + el);
+            v = valid ? x : (AT)0;
+        };
+        uint64_t coord;
+        AT v;
+        bool valid;
+        fetch(0, coord, v, valid);
+        uint32_t carry[N];  // FIB: the fiber key of the slot's last position
+        #pragma unroll
+        for (int m = 0; m < N; ++m) carry[m] = 0;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint32_t base = p0 + r * L;
+            uint32_t off[N], row = 0;
+            #pragma unroll
+            for (int m = 0; m < N; ++m) {
+                const uint64_t i = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
+                off[m] = (uint32_t)i * (uint32_t)(R * (int)sizeof(FT));
+                if (m == mode) row = (uint32_t)i;
+            }
+            uint32_t wv[RWP];
+            #pragma unroll
+            for (int k = 0; k < RWP; ++k) wv[k] = 0;
+            wv[0] = row;
+            if (FIB) {
+                // a fiber starts where the output row or an outer index changes
+                bool start = base + sub == p0;
+                #pragma unroll
+                for (int m = 0; m < N; ++m) {
+                    const uint32_t up = __shfl(off[m], sub > 0 ? lane - 1 : lane);
+                    const bool key = m != a.inner;
+                    start = start || (key && (sub > 0 ? up : carry[m]) != off[m]);
+                }
+                #pragma unroll
+                for (int m = 0; m < N; ++m) carry[m] = __shfl(off[m], slot * L + L - 1);
+                wv[0] |= (start && valid) ? 0x80000000u : 0u;
+            }
+            #pragma unroll
+            for (int j = 0; j < N - 1; ++j) {
+                #pragma unroll
+                for (int m = 0; m < N; ++m)
+                    if (m == in[j]) wv[1 + j] = off[m];
+            }
+            __builtin_memcpy(&wv[N], &v, sizeof(AT));
+            wave_lds_sync();  // the previous round's reads are done
+            #pragma unroll
+            for (int k = 0; k < RWP; k += 4)
+                *reinterpret_cast<uint4*>(my_rec + k) = uint4{wv[k], wv[k + 1], wv[k + 2], wv[k + 3]};
+            wave_lds_sync();
+            #pragma unroll
+            for (int k0 = 0; k0 < L; k0 += PB) {
+                uint32_t rv[PB][RWP];
+                AT x[PB][C], po[PB][C];
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    #pragma unroll
+                    for (int k = 0; k < RWP; k += 4) {
+                        const uint4 u = *reinterpret_cast<const uint4*>(slot_rec + (k0 + b) * RWP + k);
+                        rv[b][k] = u.x; rv[b][k + 1] = u.y; rv[b][k + 2] = u.z; rv[b][k + 3] = u.w;
+                    }
+                }
+                if (FIB) {
+                    #pragma unroll
+                    for (int b = 0; b < PB; ++b) row_piece(in[0], rv[b][1], x[b]);
+                    // outer rows, by the lanes whose position starts a fiber
+                    #pragma unroll
+                    for (int b = 0; b < PB; ++b) {
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) po[b][cc] = 1;
+                        if (rv[b][0] >> 31) {
+                            #pragma unroll
+                            for (int j = 1; j < N - 1; ++j) {
+                                AT y[C];
+                                row_piece(in[j], rv[b][1 + j], y);
+                                #pragma unroll
+                                for (int cc = 0; cc < C; ++cc) po[b][cc] *= y[cc];
+                            }
+                        }
+                    }
+                } else {
+                    // the product of the value and every input row
+                    #pragma unroll
+                    for (int b = 0; b < PB; ++b) {
+                        AT vb;
+                        __builtin_memcpy(&vb, &rv[b][N], sizeof(AT));
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) x[b][cc] = vb;
+                        #pragma unroll
+                        for (int j = 0; j < N - 1; ++j) {
+                            AT y[C];
+                            row_piece(in[j], rv[b][1 + j], y);
+                            #pragma unroll
+                            for (int cc = 0; cc < C; ++cc) x[b][cc] *= y[cc];
+                        }
+                    }
+                }
+                // the next round's nonzero, behind this batch's gathers
+                if (k0 == 0) fetch(r + 1, coord, v, valid);
+                #pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    const uint32_t rb = FIB ? rv[b][0] & 0x7fffffffu : rv[b][0];
+                    const bool fs = FIB ? (rv[b][0] >> 31) != 0 : true;
+                    if (fs && have && rb != cur) {
+                        flush();
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) acc[cc] = 0;
+                    }
+                    cur = fs ? rb : cur;
+                    have = have || fs;
+                    if (FIB) {
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) scale[cc] = fs ? po[b][cc] : scale[cc];
+                        AT vb;
+                        __builtin_memcpy(&vb, &rv[b][N], sizeof(AT));
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) acc[cc] += (vb * x[b][cc]) * scale[cc];
+                    } else {
+                        #pragma unroll
+                        for (int cc = 0; cc < C; ++cc) acc[cc] += x[b][cc];
+                    }
+                }
+            }
+        }
+        // the slots' last rows: segmented scan over the slots, as in k_mttkrp_sorted
+        const int below = lane - L, above = lane + L;
+        const uint32_t prev_cur = __shfl(cur, below < 0 ? lane : below);
+        const bool prev_have = __shfl(have, below < 0 ? lane : below);
+        bool head = slot == 0 || !have || !prev_have || prev_cur != cur;
+        #pragma unroll
+        for (int d = 1; d < S; d <<= 1) {
+            const int from = lane - d * L < 0 ? lane : lane - d * L;
+            const bool pf = __shfl(head, from);
+            AT pa[C];
+            #pragma unroll
+            for (int cc = 0; cc < C; ++cc) pa[cc] = __shfl(acc[cc], from);
+            if (slot >= d) {
+                if (!head) {
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) acc[cc] += pa[cc];
+                }
+                head = head || pf;
+            }
+        }
+        const uint32_t next_cur = __shfl(cur, above >= WAVE ? lane : above);
+        const bool next_have = __shfl(have, above >= WAVE ? lane : above);
+        const bool tail = slot == S - 1 || !next_have || next_cur != cur;
+        if (have && tail) flush();
+        wave_lds_sync();  // the next column block rewrites the records
+    }
+}
+
 // ------------------------------------------------------------- atomic
 
 template <int N, typename FT, typename VT, int C, int L, bool EXACT>
@@ -889,11 +1129,11 @@ inline const void* kernel_ptr(int which)
     if (which == W_LVL3) return (const void*)&k_mttkrp_lvl3<N, FT, VT, C, L, EX>;
     if (which == W_WINDOW) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint32_t>;
     if (which == W_WINDOW64) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint64_t>;
-    if (which == W_SORTED) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint32_t>;
+    if (which == W_SORTED) return (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, false>;
     if (which == W_SORTED64) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint64_t>;
     if (which == W_FIBER || which == W_FIBER64) {
         if constexpr (N >= 3)
-            return which == W_FIBER ? (const void*)&k_mttkrp_fiber<N, FT, VT, C, L, EX, uint32_t>
+            return which == W_FIBER ? (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, true>
                                     : (const void*)&k_mttkrp_fiber<N, FT, VT, C, L, EX, uint64_t>;
         return nullptr;  // an order-2 tensor has no fibers
     }
