@@ -148,32 +148,70 @@ __global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, 
         out[R * R + e] = ((s_ip[0][e] + s_ip[1][e]) + s_ip[2][e]) + s_ip[3][e];
 }
 
-// Sums nparts partial blocks of (R*R + R): G (R*R) and ip (R).  A workgroup
-// takes 64 entries; its 4 waves sum the parts p = w, w + 4, ... of them and
-// the 4 sums are added in wave order (a fixed order: deterministic).
-__global__ __launch_bounds__(WG) void k_gram_reduce(const double* __restrict__ part, int nparts,
-                                                    int R, double* __restrict__ G,
-                                                    double* __restrict__ ip)
+// Deterministic reduction of nparts rows of W doubles (part[p][e], row
+// major) over p, in two passes that both fill the GPU.  Pass 1: workgroup
+// (x, y) takes entries [64 x, 64 x + 64) of the rows p = y (mod gridDim.y);
+// wave w of it folds the rows y + P2 (w + 4 j), j = 0, 1, ..., with two
+// accumulators, the four waves' results are combined in wave order and
+// written over row y (only this workgroup reads row y's entries, and it
+// has read them all by then).  Pass 2 folds rows 0..P2-1 in order.  which:
+// 0 sum (seed 0), 1 max seeded with 1.
+constexpr int RED_SPLIT = 16;
+
+__device__ __forceinline__ double fold_op(int which, double a, double b) { return which == 0 ? a + b : fmax(a, b); }
+
+__global__ __launch_bounds__(WG) void k_fold_parts(double* __restrict__ part, int nparts, int W, int which)
 {
     __shared__ double s_sum[WAVES_PER_WG][WAVE];
-    const int stride = R * R + R;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int e = blockIdx.x * WAVE + lane;
-    double s0 = 0.0, s1 = 0.0;
-    if (e < stride) {
-        int p = w;
-        for (; p + WAVES_PER_WG < nparts; p += 2 * WAVES_PER_WG) {
-            s0 += part[(uint64_t)p * stride + e];
-            s1 += part[(uint64_t)(p + WAVES_PER_WG) * stride + e];
+    const int e = blockIdx.x * WAVE + lane, y = blockIdx.y, P2 = gridDim.y;
+    const double seed = which == 0 ? 0.0 : 1.0;
+    double a0 = seed, a1 = seed;
+    if (e < W) {
+        int p = y + P2 * w;
+        for (; p + WAVES_PER_WG * P2 < nparts; p += 2 * WAVES_PER_WG * P2) {
+            a0 = fold_op(which, a0, part[(uint64_t)p * W + e]);
+            a1 = fold_op(which, a1, part[(uint64_t)(p + WAVES_PER_WG * P2) * W + e]);
         }
-        if (p < nparts) s0 += part[(uint64_t)p * stride + e];
+        if (p < nparts) a0 = fold_op(which, a0, part[(uint64_t)p * W + e]);
     }
-    s_sum[w][lane] = s0 + s1;
+    s_sum[w][lane] = fold_op(which, a0, a1);
     __syncthreads();
-    if (w == 0 && e < stride) {
-        const double s = ((s_sum[0][lane] + s_sum[1][lane]) + s_sum[2][lane]) + s_sum[3][lane];
-        if (e < R * R) G[e] = s;
-        else if (ip) ip[e - R * R] = s;
+    if (w == 0 && e < W)
+        part[(uint64_t)y * W + e] =
+            fold_op(which, fold_op(which, fold_op(which, s_sum[0][lane], s_sum[1][lane]), s_sum[2][lane]), s_sum[3][lane]);
+}
+
+static int fold_parts(double* part, int nparts, int W, int which, int* rows_left, hipStream_t st)
+{
+    const int p2 = nparts < RED_SPLIT ? (nparts < 1 ? 1 : nparts) : RED_SPLIT;
+    hipLaunchKernelGGL(k_fold_parts, dim3((W + WAVE - 1) / WAVE, p2), dim3(WG), 0, st, part, nparts, W, which);
+    HIP_TRY(hipGetLastError());
+    *rows_left = p2;
+    return BLCO_OK;
+}
+
+// Pass 2 of the Gram reduction: G (R*R) and ip (R) from the P2 folded rows
+// of (R*R + R).  With inv (the new lambda's inverse), G is that of the
+// factor scaled by inv, G_ij = inv_i inv_j sum, and ip_r = inv_r c_r sum,
+// where c (colscale, may be null: ones) is the column scale the MTTKRP
+// result carries relative to that of the normalised factors.
+__global__ __launch_bounds__(WAVE) void k_gram_finish(const double* __restrict__ part, int p2, int R,
+                                                      const double* __restrict__ inv,
+                                                      const double* __restrict__ colscale,
+                                                      double* __restrict__ G, double* __restrict__ ip)
+{
+    const int e = blockIdx.x * WAVE + threadIdx.x, W = R * R + R;
+    if (e >= W) return;
+    double s = part[e];
+    for (int y = 1; y < p2; ++y) s += part[(uint64_t)y * W + e];
+    if (e < R * R) {
+        if (inv) s *= inv[e / R] * inv[e % R];
+        G[e] = s;
+    } else if (ip) {
+        const int r = e - R * R;
+        if (inv) s *= inv[r] * (colscale ? colscale[r] : 1.0);
+        ip[r] = s;
     }
 }
 
@@ -188,11 +226,22 @@ __global__ __launch_bounds__(WG) void k_gram_reduce(const double* __restrict__ p
 __global__ __launch_bounds__(WG) void k_factorize(GramSet gs, int nmodes, int mode, int R,
                                                   double eps, double* __restrict__ fac,
                                                   double* __restrict__ work,
-                                                  int* __restrict__ status)
+                                                  int* __restrict__ status,
+                                                  const double* __restrict__ dinv,
+                                                  double* __restrict__ colscale)
 {
     extern __shared__ double s_a[];  // R x R: V, then L (lower); then R x R X = L^{-1}
     double* s_x = s_a + R * R;
     const int tid = threadIdx.x;
+    // the column scale of this mode's MTTKRP result: the product of the other
+    // factors' pending inverse scales (their stored columns carry lambda)
+    if (colscale)
+        for (int k = tid; k < R; k += WG) {
+            double c = 1.0;
+            for (int m = 0; m < nmodes; ++m)
+                if (m != mode) c *= dinv[m * R + k];
+            colscale[k] = c;
+        }
     for (int e = tid; e < R * R; e += WG) {
         double v = 1.0;
         for (int m = 0; m < nmodes; ++m)
@@ -437,7 +486,8 @@ __global__ __launch_bounds__(WG) void k_pinv_general(double* __restrict__ A, int
 template <typename MT, typename UT, int G>
 __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT* __restrict__ U,
                                                    uint64_t rows, int R,
-                                                   const double* __restrict__ fac, int which,
+                                                   const double* __restrict__ fac,
+                                                   const double* __restrict__ colscale, int which,
                                                    double* __restrict__ stats)
 {
     constexpr int SLOTS = WG / G;
@@ -445,9 +495,11 @@ __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT*
     __shared__ double s_st[SLOTS][G];
     const int j = threadIdx.x % G, slot = threadIdx.x / G;
     const bool col = j < R;
+    // row j of F diag(c): the MTTKRP result's columns carry c's inverse
     double f[G];
     #pragma unroll
-    for (int k = 0; k < G; ++k) f[k] = (col && k < R) ? fac[j * R + k] : 0.0;
+    for (int k = 0; k < G; ++k)
+        f[k] = (col && k < R) ? fac[j * R + k] * (colscale ? colscale[k] : 1.0) : 0.0;
     double st = which == 0 ? 0.0 : 1.0;
     const uint64_t step = (uint64_t)gridDim.x * SLOTS;
     // every lane runs the same number of iterations: the barriers below are uniform
@@ -475,34 +527,22 @@ __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT*
     }
 }
 
-// lambda and 1/lambda from per-workgroup statistics: lane t sums the parts
-// p = t / R, t / R + WG / R, ... of column t % R; the partial results are
-// combined in a fixed order (deterministic).
-__global__ __launch_bounds__(WG) void k_lambda(const double* __restrict__ stats, int nparts, int R,
-                                               int which, double* __restrict__ lambda,
-                                               double* __restrict__ inv)
+// lambda and 1/lambda (0 for a zero column) from the P2 folded rows of the
+// column statistics; dinv (may be null) receives 1/lambda too.
+__global__ __launch_bounds__(WAVE) void k_lambda_finish(const double* __restrict__ stats, int p2, int R,
+                                                        int which, double* __restrict__ lambda,
+                                                        double* __restrict__ inv,
+                                                        double* __restrict__ dinv)
 {
-    __shared__ double s_x[WG];
-    const int per = R <= WG ? WG / R : 1;  // lanes per column
-    const int r = threadIdx.x % R, q = threadIdx.x / R;
-    double x = (which == 0) ? 0.0 : 1.0;
-    if (q < per && r < R)
-        for (int p = q; p < nparts; p += per) {
-            const double y = stats[(uint64_t)p * R + r];
-            x = (which == 0) ? x + y : fmax(x, y);
-        }
-    s_x[threadIdx.x] = x;
-    __syncthreads();
-    for (int c = threadIdx.x; c < R; c += WG) {
-        double y = (which == 0) ? 0.0 : 1.0;
-        for (int k = 0; k < per && k * R + c < WG; ++k) {
-            const double z = s_x[k * R + c];
-            y = (which == 0) ? y + z : fmax(y, z);
-        }
-        if (which == 0) y = sqrt(y);
-        lambda[c] = y;
-        inv[c] = (y == 0.0) ? 0.0 : 1.0 / y;
-    }
+    const int c = blockIdx.x * WAVE + threadIdx.x;
+    if (c >= R) return;
+    double y = stats[c];
+    for (int q = 1; q < p2; ++q) y = fold_op(which, y, stats[(uint64_t)q * R + c]);
+    if (which == 0) y = sqrt(y);
+    const double iv = (y == 0.0) ? 0.0 : 1.0 / y;
+    lambda[c] = y;
+    inv[c] = iv;
+    if (dinv) dinv[c] = iv;
 }
 
 // ------------------------------------------------------------------- fit
@@ -646,8 +686,9 @@ int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg)
     return BLCO_OK;
 }
 
-int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
-                double* part, double* G, double* ip, hipStream_t st)
+static int gram_any(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
+                    double* part, double* G, double* ip, const double* post_inv, const double* colscale,
+                    hipStream_t st)
 {
     if (R < 1 || R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] for the Gram kernel", R);
     int nparts;
@@ -658,10 +699,26 @@ int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scal
     else if (dtype == BLCO_F32) rc = gram_t<float>((float*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
     else rc = gram_t<bf16>((bf16*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
     if (rc) return rc;
-    int stride = R * R + R;
-    hipLaunchKernelGGL(k_gram_reduce, dim3((stride + WAVE - 1) / WAVE), dim3(WG), 0, st, part, nparts, R, G, ip);
+    const int W = R * R + R;
+    int p2 = 0;
+    rc = fold_parts(part, nparts, W, 0, &p2, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gram_finish, dim3((W + WAVE - 1) / WAVE), dim3(WAVE), 0, st, part, p2, R, post_inv,
+                       colscale, G, ip);
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
+}
+
+int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
+                double* part, double* G, double* ip, hipStream_t st)
+{
+    return gram_any(U, dtype, rows, R, inv_scale, M, part, G, ip, nullptr, nullptr, st);
+}
+
+int launch_gram_of_scaled(const void* U, int dtype, uint64_t rows, int R, const double* inv, const void* M,
+                          const double* colscale, double* part, double* G, double* ip, hipStream_t st)
+{
+    return gram_any(const_cast<void*>(U), dtype, rows, R, nullptr, M, part, G, ip, inv, colscale, st);
 }
 
 double gram_eps(int factor_dtype)
@@ -670,19 +727,20 @@ double gram_eps(int factor_dtype)
 }
 
 int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps, double* fac,
-                     double* work, int* status, hipStream_t st)
+                     double* work, int* status, hipStream_t st, const double* dinv, double* colscale)
 {
     if (R < 1 || R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] in the factorisation", R);
     size_t lds = sizeof(double) * 2 * (size_t)R * R;
     if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute((const void*)k_factorize,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_factorize, dim3(1), dim3(WG), lds, st, gs, nmodes, mode, R, eps, fac, work, status);
+    hipLaunchKernelGGL(k_factorize, dim3(1), dim3(WG), lds, st, gs, nmodes, mode, R, eps, fac, work, status,
+                       dinv, colscale);
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }
 
 template <typename MT, typename UT>
-static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac,
+static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac, const double* cs,
                         int which, double* stats, hipStream_t st)
 {
     if (rows == 0) return BLCO_OK;
@@ -690,21 +748,23 @@ static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* 
     if (R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d > 64 in the row solve", R);
     const int g = R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64;
     switch (g) {
-        case 8: hipLaunchKernelGGL((k_solve_rows<MT, UT, 8>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
-        case 16: hipLaunchKernelGGL((k_solve_rows<MT, UT, 16>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
-        case 32: hipLaunchKernelGGL((k_solve_rows<MT, UT, 32>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
-        default: hipLaunchKernelGGL((k_solve_rows<MT, UT, 64>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        case 8: hipLaunchKernelGGL((k_solve_rows<MT, UT, 8>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
+        case 16: hipLaunchKernelGGL((k_solve_rows<MT, UT, 16>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
+        case 32: hipLaunchKernelGGL((k_solve_rows<MT, UT, 32>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
+        default: hipLaunchKernelGGL((k_solve_rows<MT, UT, 64>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
     }
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }
 
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
-                      int which, double* stats, hipStream_t st)
+                      int which, double* stats, hipStream_t st, const double* colscale)
 {
-    if (dtype == BLCO_F64) return solve_rows_t<double, double>((const double*)M, (double*)U, rows, R, fac, which, stats, st);
-    if (dtype == BLCO_F32) return solve_rows_t<float, float>((const float*)M, (float*)U, rows, R, fac, which, stats, st);
-    return solve_rows_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, which, stats, st);
+    if (dtype == BLCO_F64)
+        return solve_rows_t<double, double>((const double*)M, (double*)U, rows, R, fac, colscale, which, stats, st);
+    if (dtype == BLCO_F32)
+        return solve_rows_t<float, float>((const float*)M, (float*)U, rows, R, fac, colscale, which, stats, st);
+    return solve_rows_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, colscale, which, stats, st);
 }
 
 // Workgroups of the row solve (each walks rows with a grid stride).
@@ -723,10 +783,14 @@ int launch_pinv_general(double* A, int n, int* info, hipStream_t st)
     return BLCO_OK;
 }
 
-int launch_lambda(const double* stats, int nparts, int R, int which, double* lambda, double* inv,
-                  hipStream_t st)
+int launch_lambda(double* stats, int nparts, int R, int which, double* lambda, double* inv, hipStream_t st,
+                  double* dinv)
 {
-    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(WG), 0, st, stats, nparts, R, which, lambda, inv);
+    int p2 = 0;
+    int rc = fold_parts(stats, nparts, R, which, &p2, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_lambda_finish, dim3((R + WAVE - 1) / WAVE), dim3(WAVE), 0, st, stats, p2, R, which, lambda,
+                       inv, dinv);
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }

@@ -20,22 +20,32 @@ int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg);
 // gram_parts(rows) * (R*R + R) doubles.
 int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
                 double* part, double* G, double* ip, hipStream_t st);
+// G and ip of the factor U diag(inv) without writing it: G = diag(inv) U^T U
+// diag(inv), ip[col] = inv[col] colscale[col] sum_i U_ic M_ic (colscale may
+// be null: ones).  part as for launch_gram.
+int launch_gram_of_scaled(const void* U, int dtype, uint64_t rows, int R, const double* inv, const void* M,
+                          const double* colscale, double* part, double* G, double* ip, hipStream_t st);
 // Machine epsilon of the precision the Grams of factor_dtype factors are
 // accumulated in (f64 for f64 factors, f32 for f32 and bf16 ones).
 double gram_eps(int factor_dtype);
 // fac: R x R (V^{-1}, or V^+ with status 1); work: 2 R^2 + R doubles.  eps
-// scales the Cholesky pivot floor and the pseudo-inverse tolerance.
+// scales the Cholesky pivot floor and the pseudo-inverse tolerance.  With
+// dinv (nmodes x R) colscale[k] receives prod_{m != mode} dinv[m][k].
 int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps, double* fac,
-                     double* work, int* status, hipStream_t st);
+                     double* work, int* status, hipStream_t st, const double* dinv = nullptr,
+                     double* colscale = nullptr);
 // In-place pseudo-inverse of a general n x n matrix (n <= 64); info
 // (device, may be null): 0 converged, else the sweep count.
 int launch_pinv_general(double* A, int n, int* info, hipStream_t st);
 uint32_t solve_parts(uint64_t rows);
-// stats: solve_parts(rows) * R doubles.
+// U = diag-scaled solve: row i of U = (c o M_i) F^T with c = colscale (may
+// be null: ones).  stats: solve_parts(rows) * R doubles.
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
-                      int which, double* stats, hipStream_t st);
-int launch_lambda(const double* stats, int nparts, int R, int which, double* lambda, double* inv,
-                  hipStream_t st);
+                      int which, double* stats, hipStream_t st, const double* colscale = nullptr);
+// lambda, 1/lambda (and into dinv when non-null) from the row solve's
+// statistics (folded in place).
+int launch_lambda(double* stats, int nparts, int R, int which, double* lambda, double* inv, hipStream_t st,
+                  double* dinv = nullptr);
 // out[0] = fit, out[1] = inner product, out[2] = ||M||^2.
 int launch_fit(const GramSet& gs, int nmodes, int R, const double* lambda, const double* ip,
                double normX2, double* out, hipStream_t st);

@@ -11,6 +11,15 @@
 //   fit = 1 - ||X - [[lambda; U]]|| / ||X||, stop when it > 0 and
 //   |fit - fit_old| < tolerance.
 //
+// The division by lambda is not written back: each factor is stored as the
+// solve leaves it, U_n diag(lambda_n), with 1 / lambda_n kept aside (dinv).
+// Its Gram is formed from the stored factor and scaled by 1 / lambda on
+// both sides; the MTTKRP of a later mode then carries the product c of the
+// other factors' lambdas in its columns, and that mode's solve multiplies
+// M by c^{-1} (folded into F).  Mathematically the same step; each mode's
+// update passes over U_n once (written by the solve, read by the Gram)
+// instead of rewriting it in place.  blco_cpals_get applies 1 / lambda.
+//
 // Everything stays on one stream with no host round trip inside a sweep;
 // the host reads back the fit (8 bytes) once per iteration.  With use_graph
 // one sweep (after iteration 0, whose norm differs) is captured into a
@@ -74,7 +83,7 @@ int enqueue_factorize(blco_cpals_state* s, int n, hipStream_t st)
     for (int m = 0; m < s->N; ++m) gs.g[m] = s->G[m];
     // the pivot floor of f64 unless the caller asked for the factors' precision
     const double eps = s->o.solve_floor ? gram_eps(s->ft) : gram_eps(BLCO_F64);
-    return launch_factorize(gs, s->N, n, s->R, eps, s->fac, s->work, s->status, st);
+    return launch_factorize(gs, s->N, n, s->R, eps, s->fac, s->work, s->status, st, s->dinv, s->colscale);
 }
 
 int enqueue_step(blco_cpals_state* s, int n, int which_norm, hipStream_t st)
@@ -88,10 +97,13 @@ int enqueue_update(blco_cpals_state* s, int n, int which_norm, hipStream_t st)
     const int N = s->N, R = s->R;
     const uint64_t I = s->d->dims[n];
     const int last = s->order[N - 1];
-    int rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, st);
-    if (!rc) rc = launch_lambda(s->stats, (int)solve_parts(I), R, which_norm, s->lambda, s->inv, st);
-    if (!rc) rc = launch_gram(s->U[n], s->ft, I, R, s->inv, n == last ? s->Mbuf : nullptr, s->gram_part,
-                              s->G[n], s->ip, st);
+    int rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, st, s->colscale);
+    if (!rc)
+        rc = launch_lambda(s->stats, (int)solve_parts(I), R, which_norm, s->lambda, s->inv, st,
+                           s->dinv + (size_t)n * R);
+    if (!rc)
+        rc = launch_gram_of_scaled(s->U[n], s->ft, I, R, s->inv, n == last ? s->Mbuf : nullptr, s->colscale,
+                                   s->gram_part, s->G[n], s->ip, st);
     return rc;
 }
 
@@ -156,7 +168,7 @@ void blco_cpals_free(blco_cpals_state* s)
     }
     if (s->side) (void)hipStreamDestroy(s->side);
     void* bufs[] = {s->Mbuf, s->lambda, s->inv, s->fac, s->work, s->status, s->gram_part,
-                    s->stats, s->ip, s->fitbuf, s->norm_part};
+                    s->stats, s->ip, s->fitbuf, s->norm_part, s->dinv, s->colscale};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (s->h_fit) (void)hipHostFree(s->h_fit);
@@ -231,7 +243,9 @@ int blco_cpals_create(blco_dev* d, const blco_cpals_opts* o, const double* const
         hipMalloc((void**)&s->stats, sizeof(double) * (size_t)(sparts ? sparts : 1) * R) != hipSuccess ||
         hipMalloc((void**)&s->ip, sizeof(double) * R) != hipSuccess ||
         hipMalloc((void**)&s->fitbuf, sizeof(double) * 4) != hipSuccess ||
-        hipMalloc((void**)&s->norm_part, sizeof(double) * (nnorm + 1)) != hipSuccess)
+        hipMalloc((void**)&s->norm_part, sizeof(double) * (nnorm + 1)) != hipSuccess ||
+        hipMalloc((void**)&s->dinv, sizeof(double) * (size_t)N * R) != hipSuccess ||
+        hipMalloc((void**)&s->colscale, sizeof(double) * R) != hipSuccess)
         return bad(fail(BLCO_ENOMEM, "solver work allocation failed"));
     if (hipHostMalloc((void**)&s->h_fit, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
         return bad(fail(BLCO_ENOMEM, "pinned allocation failed"));
@@ -239,6 +253,7 @@ int blco_cpals_create(blco_dev* d, const blco_cpals_opts* o, const double* const
         return bad(fail(BLCO_EHIP, "event creation failed"));
     // lambda = 1 (KruskalModelRandomInit), ||X||^2, initial Grams
     int rc = launch_fill(s->lambda, R, BLCO_F64, 1.0, s->st);
+    if (!rc) rc = launch_fill(s->dinv, (uint64_t)N * R, BLCO_F64, 1.0, s->st);
     if (rc) return bad(rc);
     if (d->nnz && !d->stream_data) {
         rc = launch_sumsq(d->d_vals, d->vdtype, d->nnz, s->norm_part, nnorm, s->norm_part + nnorm, s->st);
@@ -335,14 +350,19 @@ int blco_cpals_get(blco_cpals_state* s, double* const* U, double* lambda)
 {
     if (!s) return fail(BLCO_EINVAL, "null solver");
     HIP_TRY(hipStreamSynchronize(s->st));
-    const int fsz = dtype_size(s->ft);
+    const int fsz = dtype_size(s->ft), R = s->R;
     std::vector<char> stage;
+    std::vector<double> dinv((size_t)s->N * R);
+    HIP_TRY(hipMemcpy(dinv.data(), s->dinv, sizeof(double) * dinv.size(), hipMemcpyDeviceToHost));
     for (int m = 0; m < s->N; ++m) {
         if (!U || !U[m]) continue;
-        uint64_t n = s->d->dims[m] * (uint64_t)s->R;
+        uint64_t n = s->d->dims[m] * (uint64_t)R;
         stage.resize(n * fsz);
         HIP_TRY(hipMemcpy(stage.data(), s->U[m], n * fsz, hipMemcpyDeviceToHost));
         from_dtype(stage.data(), U[m], n, s->ft);
+        // the stored factor carries lambda_m in its columns
+        const double* iv = dinv.data() + (size_t)m * R;
+        for (uint64_t e = 0; e < n; ++e) U[m][e] *= iv[e % R];
     }
     if (lambda) HIP_TRY(hipMemcpy(lambda, s->lambda, sizeof(double) * s->R, hipMemcpyDeviceToHost));
     return BLCO_OK;

@@ -19,6 +19,11 @@ struct blco_cpals_state {
     double *lambda = nullptr, *inv = nullptr, *fac = nullptr, *work = nullptr;
     int* status = nullptr;
     double *gram_part = nullptr, *stats = nullptr, *ip = nullptr, *fitbuf = nullptr;
+    // the factors are stored unnormalised: U[m] = (normalised factor) diag(lambda_m);
+    // dinv[m * R + r] = 1 / lambda_m[r] (0 for a zero column; 1 before mode m's
+    // first update), and colscale the product of the other modes' dinv for
+    // the mode being updated
+    double *dinv = nullptr, *colscale = nullptr;
     double* norm_part = nullptr;
     double* h_fit = nullptr;  // pinned
     double normX2 = 0.0;
