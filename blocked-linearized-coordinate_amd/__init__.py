@@ -102,6 +102,8 @@ def lib():
             "blco_dev_create_budget": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_uint64, C.POINTER(vp)]),
             "blco_dev_residency": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                              C.POINTER(C.c_uint64)]),
+            "blco_dev_rows_plan": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
             "blco_dev_free": (None, [vp]),
             "blco_generate_device": (C.c_int, [C.POINTER(GenSpec), C.c_uint64, C.c_int, C.POINTER(vp)]),
             "blco_dev_to_host": (C.c_int, [vp, C.POINTER(vp)]),
@@ -486,6 +488,13 @@ class DeviceTensor:
         self.nmodes, self.dims, self.nnz, self.nblocks = n.value, dims[:n.value].copy(), nnz.value, nb.value
         self.value_dtype = value_dtype
         return self
+
+    def rows_plan(self, mode, rank, factor_dtype=F64):
+        """(chunks per wave, interleaved, LDS hot rows) of kernel 14 on a mode
+        after it has run there."""
+        c, i, h = C.c_uint32(0), C.c_int(0), C.c_uint64(0)
+        _check(lib().blco_dev_rows_plan(self.h, mode, rank, factor_dtype, C.byref(c), C.byref(i), C.byref(h)))
+        return c.value, bool(i.value), h.value
 
     def residency(self):
         """(resident nonzeros, streamed nonzeros, device bytes)."""

@@ -727,6 +727,107 @@ static int launch_range(blco_dev* d, int which, MttkrpArgs a, const ChunkTab* t,
     return launch_mttkrp(which, a, ftype, grid, lds, st);
 }
 
+static size_t plan_rows(blco_dev* d, int mode, int rank, int ftype, bool fib, MttkrpArgs* a);
+
+// The best window of h consecutive rows of mode m (most nonzeros); its
+// start, and its nonzero count in *covered.
+static uint64_t best_window(blco_dev* d, int m, uint64_t h, uint64_t* covered)
+{
+    std::vector<uint64_t>& pre = d->idx_prefix[m];
+    *covered = 0;
+    if (pre.empty()) {
+        std::vector<uint64_t> cnt;
+        if (dev_split_counts(d, m, &cnt)) return 0;  // no window when the counts fail
+        pre.assign(cnt.size() + 1, 0);
+        for (size_t i = 0; i < cnt.size(); ++i) pre[i + 1] = pre[i] + cnt[i];
+    }
+    const uint64_t n = pre.size() - 1;
+    if (h > n) h = n;
+    uint64_t best = 0, lo = 0;
+    for (uint64_t i = 0; i + h <= n; ++i) {
+        const uint64_t c = pre[i + h] - pre[i];
+        if (c > best) { best = c; lo = i; }
+    }
+    *covered = best;
+    return lo;
+}
+
+// Chunks per wave and the LDS windows of hot rows for the row-ordered
+// kernels (k_mttkrp_rows), into a; returns the dynamic LDS bytes.  Cached
+// per (mode, rank, factor type, form, chunk count).
+size_t rows_kernel_plan(blco_dev* d, int mode, int rank, int ftype, bool fib, MttkrpArgs* a)
+{
+    for (const RowsPlan& p : d->rows_plans[mode])
+        if (p.rank == rank && p.ftype == ftype && p.fib == fib && p.nchunks == a->nchunks) {
+            a->cpw = p.cpw;
+            a->cpw_interleave = p.interleave;
+            for (int t = 0; t < 2; ++t) {
+                a->hot_lo[t] = p.hot_lo[t];
+                a->hot_n[t] = p.hot_n[t];
+            }
+            return p.lds;
+        }
+    const size_t lds = plan_rows(d, mode, rank, ftype, fib, a);
+    d->rows_plans[mode].push_back(RowsPlan{rank, ftype, fib, a->nchunks, a->cpw, a->cpw_interleave,
+                                           {a->hot_lo[0], a->hot_lo[1]}, {a->hot_n[0], a->hot_n[1]}, lds});
+    return lds;
+}
+
+// The windows cover, for the first two input modes (the inner and first
+// outer mode with fibers, 2/3 and 1/3 of HOT_TABLE_BYTES' rows; else
+// halves), the mode's consecutive rows with the most nonzeros; a window that
+// would hold fewer than 2% of the nonzeros is left out.  A workgroup fills
+// its windows once, so its waves take several chunks (as many as leave 1024
+// workgroups, at least half of the cap): up to 4 consecutive ones each in
+// the plain form, for rows of at most 128 bytes (FP64 rows of 256 bytes
+// were slower with windows); in the fiber form up to 8, with the grid walking the
+// chunks in step (chunk j of a wave is j grid widths on), and only when the
+// inner window holds a fifth of the nonzeros and the copy is not split into
+// XCD tiles -- the forms and counts that were faster on the 77M-nonzero
+// and 1.7B-nonzero workloads (a wave per chunk, no windows, otherwise).
+static size_t plan_rows(blco_dev* d, int mode, int rank, int ftype, bool fib, MttkrpArgs* a)
+{
+    const uint32_t n = a->nchunks;
+    a->cpw = 1;
+    a->cpw_interleave = 0;
+    a->hot_lo[0] = a->hot_lo[1] = 0;
+    a->hot_n[0] = a->hot_n[1] = 0;
+    const uint64_t row_bytes = (uint64_t)rank * dtype_size(ftype);
+    // up to 8 (fibers) or 4 chunks per wave while 1024 workgroups remain
+    const uint32_t cap = fib ? 8 : 4, fit = n / (1024u * WAVES_PER_WG);
+    const uint32_t cpw = fit < cap ? fit : cap;
+    // windows need whole 16-byte pieces and at least half the chunks per wave
+    if (row_bytes % 16 != 0 || row_bytes > HOT_TABLE_BYTES / 8 || cpw < cap / 2) return 0;
+    if (fib && d->stiles[mode] > 1) return 0;
+    if (!fib && row_bytes > 128) return 0;
+    int in[2] = {-1, -1}, k = 0;
+    if (fib) in[k++] = a->inner;
+    for (int m = 0; m < d->nmodes && k < 2; ++m)
+        if (m != mode && !(fib && m == a->inner)) in[k++] = m;
+    const uint64_t total = HOT_TABLE_BYTES / row_bytes;
+    const uint64_t h0 = k == 1 ? total : fib ? total * 2 / 3 : total / 2;
+    const uint64_t want[2] = {h0, total - h0};
+    uint64_t lo[2] = {0, 0}, h[2] = {0, 0}, covered[2] = {0, 0};
+    for (int t = 0; t < k; ++t) {
+        h[t] = std::min<uint64_t>(want[t], d->dims[in[t]]);
+        lo[t] = best_window(d, in[t], h[t], &covered[t]);
+    }
+    if (fib && covered[0] * 5 < d->nnz) return 0;
+    a->cpw = cpw;
+    a->cpw_interleave = fib ? 1 : 0;
+    size_t lds = 0;
+    for (int t = 0; t < k; ++t) {
+        if (h[t] == 0 || covered[t] * 50 < d->nnz) continue;
+        a->hot_lo[t] = (uint32_t)lo[t];
+        a->hot_n[t] = (uint32_t)h[t];
+        lds += h[t] * row_bytes;
+        progress("row-ordered kernel of mode %d: rows [%llu, %llu) of mode %d in LDS, %.1f%% of the nonzeros",
+                 mode, (unsigned long long)lo[t], (unsigned long long)(lo[t] + h[t]), in[t],
+                 100.0 * covered[t] / d->nnz);
+    }
+    return lds;
+}
+
 int mttkrp_prepare(blco_dev* d, int mode, int rank, int ftype, int kernel)
 {
     if (!d || mode < 0 || mode >= d->nmodes || !d->nnz || d->stream_data) return BLCO_OK;
@@ -949,7 +1050,16 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
             kind = wide ? W_FIBER64 : W_FIBER;
             a.inner = d->sinner[mode];
         }
-        rc = launch_range(d, kind, a, stab, 0, stab->n, ftype, st);
+        if (kind == W_SORTED || kind == W_FIBER) {
+            a.chunks = stab->d;
+            a.chunk0 = 0;
+            a.nchunks = stab->n;
+            const size_t lds = rows_kernel_plan(d, mode, rank, ftype, kind == W_FIBER, &a);
+            const uint32_t per_wg = WAVES_PER_WG * a.cpw;
+            if (stab->n) rc = launch_mttkrp(kind, a, ftype, (stab->n + per_wg - 1) / per_wg, lds, st);
+        } else {
+            rc = launch_range(d, kind, a, stab, 0, stab->n, ftype, st);
+        }
     } else if (!d->stream_data) {
         a.coords = d->d_coords;
         a.vals = d->d_vals;
@@ -1296,6 +1406,21 @@ int blco_dev_create_budget(const blco_host* h, int value_dtype, uint64_t device_
                                 h->vals.data(), h->block_nnz.size(), h->block_start.data(), h->block_nnz.data(),
                                 h->block_coords.data(), value_dtype, 0, slot_nnz, device_budget, nullptr, nullptr,
                                 out);
+}
+
+int blco_dev_rows_plan(const blco_dev* d, int mode, int rank, int factor_dtype, uint32_t* chunks_per_wave,
+                       int* interleaved, uint64_t* lds_rows)
+{
+    if (!d || mode < 0 || mode >= d->nmodes || !chunks_per_wave || !interleaved || !lds_rows)
+        return fail(BLCO_EINVAL, "bad plan query");
+    for (const RowsPlan& p : d->rows_plans[mode])
+        if (p.rank == rank && p.ftype == factor_dtype) {
+            *chunks_per_wave = p.cpw;
+            *interleaved = (int)p.interleave;
+            *lds_rows = (uint64_t)p.hot_n[0] + p.hot_n[1];
+            return BLCO_OK;
+        }
+    return fail(BLCO_EINVAL, "mode %d has run no row-ordered MTTKRP at rank %d", mode, rank);
 }
 
 int blco_dev_residency(const blco_dev* d, uint64_t* resident_nnz, uint64_t* streamed_nnz, uint64_t* device_bytes)

@@ -115,6 +115,10 @@ struct blco_dev {
     bool sfiber[BLCO_MAX_MODES] = {};
     std::vector<uint64_t> srun_h[BLCO_MAX_MODES];
     std::list<blco::ChunkTab> stabs[BLCO_MAX_MODES];  // row-aware chunks of the copy, per tpw
+    // nonzeros per index of every mode, as prefix sums (dims + 1 entries),
+    // for the hot-row windows of the row-ordered kernels (built on first use)
+    std::vector<uint64_t> idx_prefix[BLCO_MAX_MODES];
+    std::vector<blco::RowsPlan> rows_plans[BLCO_MAX_MODES];
     void* part = nullptr;  // partial rows of the row-window MTTKRP
     size_t part_bytes = 0;
     // selector features (rank- and type-independent entries) and decisions
@@ -165,5 +169,6 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
 // kernel) would do on its first call (kernel 14's row-ordered copy), without
 // a launch; a copy that does not fit leaves the mode to the row window.
 int mttkrp_prepare(blco_dev* d, int mode, int rank, int ftype, int kernel);
+size_t rows_kernel_plan(blco_dev* d, int mode, int rank, int ftype, bool fib, MttkrpArgs* a);
 
 }  // namespace blco

@@ -56,14 +56,33 @@ struct MttkrpArgs {
     void* part;                // partial rows (rank wide), reduced by k_window_reduce
     // fiber kernel only: the input mode gathered per nonzero
     int inner;
+    // row-ordered kernels: chunks per wave (consecutive, or with
+    // cpw_interleave one grid width apart), and for the first two input
+    // modes (the inner and first outer mode with fibers) a window of rows
+    // [hot_lo, hot_lo + hot_n) kept in LDS (hot_n = 0: none)
+    uint32_t cpw;
+    uint32_t cpw_interleave;
+    uint32_t hot_lo[2];
+    uint32_t hot_n[2];
+};
+
+// Launch plan of a row-ordered kernel for one (rank, factor type, form,
+// chunk count) of a mode: chunks per wave and the hot-row windows.
+struct RowsPlan {
+    int rank, ftype;
+    bool fib;
+    uint32_t nchunks, cpw, interleave, hot_lo[2], hot_n[2];
+    size_t lds;
 };
 
 enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4, W_SORTED = 5, W_SORTED64 = 6,
        W_FIBER = 7, W_FIBER64 = 8 };
 
-// The row-ordered and fiber kernels take 32-bit byte offsets into factors
-// of fewer bytes than this (a row's offset plus its column bytes then fits).
-constexpr uint32_t FACTOR_BYTES_MAX = 0xFFFFFF00u;
+// The row-ordered and fiber kernels take 31-bit byte offsets into factors
+// of fewer bytes than this (bit 31 marks an offset into an LDS window).
+constexpr uint32_t FACTOR_BYTES_MAX = 0x80000000u;
+// LDS for the hot-row windows of the row-ordered kernels (per workgroup)
+constexpr uint32_t HOT_TABLE_BYTES = 24 * 1024;
 constexpr uint32_t FINE_CHUNK = 1024;        // nonzeros per range-table entry
 constexpr uint32_t WINDOW_LDS_BYTES = 40 * 1024;  // hot area + row window per workgroup
 constexpr int WIN_WAVES = 8;                      // waves per row-window workgroup

@@ -845,15 +845,12 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
     // position then start in different LDS banks
     constexpr int SLOT_W = L * RWP + 4;
     __shared__ __attribute__((aligned(16))) uint32_t s_rec[WAVES_PER_WG][S * SLOT_W];
+    // the hot-row windows of the first two input modes (dynamic LDS), each a
+    // contiguous run of factor rows copied as it lies in memory
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_tab[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t c = a.chunk0 + xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
-    if (c >= a.chunk0 + a.nchunks) return;  // whole wave leaves; no block barriers below
-    const Chunk ch = a.chunks[c];
-    uint64_t bc[N];
-    #pragma unroll
-    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
-    const int slot = lane / L, sub = lane % L;
     const int R = a.rank, mode = a.mode;
+    const uint32_t row_bytes = (uint32_t)R * (uint32_t)sizeof(FT);
     // input modes: FIB, the inner one first, then the outer ones
     int in[N - 1] = {};
     {
@@ -863,9 +860,23 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
         for (int m = 0; m < N; ++m)
             if (m != mode && !(FIB && m == a.inner) && j < N - 1) in[j++] = m;
     }
-    const uint32_t run = (ch.len + S - 1) / S;
-    const uint32_t p0 = min((uint32_t)slot * run, ch.len), p1 = min(p0 + run, ch.len);
-    const uint32_t rounds = (run + L - 1) / L;  // the same for every slot
+    constexpr int NT = N - 1 < 2 ? N - 1 : 2;  // input modes with a window
+    uint32_t tab_off[NT];
+    {
+        uint32_t o = 0;
+        #pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            tab_off[t] = o;
+            const uint32_t nb = a.hot_n[t] * row_bytes;  // a multiple of 16 (host-checked)
+            const unsigned char* src = reinterpret_cast<const unsigned char*>(a.f[in[t]]) +
+                                       (uint64_t)a.hot_lo[t] * row_bytes;
+            for (uint32_t i = threadIdx.x * 16u; i < nb; i += WG * 16u)
+                *reinterpret_cast<uint4*>(s_tab + o + i) = *reinterpret_cast<const uint4*>(src + i);
+            o += nb;
+        }
+    }
+    __syncthreads();  // the only block barrier: waves leave independently below
+    const int slot = lane / L, sub = lane % L;
     AT* out = reinterpret_cast<AT*>(a.out);
     const VT* vals = reinterpret_cast<const VT*>(a.vals);
     __amdgpu_buffer_rsrc_t rs[N];
@@ -873,6 +884,22 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
     for (int m = 0; m < N; ++m) rs[m] = factor_rsrc(a.f[m]);
     uint32_t* my_rec = s_rec[w] + slot * SLOT_W + sub * RWP;
     const uint32_t* slot_rec = s_rec[w] + slot * SLOT_W;
+    // the wave's chunks: a contiguous run of cpw chunks of the workgroup's
+    // range (logical workgroups of one XCD take consecutive ranges)
+    const uint32_t lw = xcd_remap(blockIdx.x, gridDim.x) * WAVES_PER_WG + w;
+    for (uint32_t cj = 0; cj < a.cpw; ++cj) {
+    // interleaved: the whole grid walks the chunks in step (the chunks in
+    // flight stay neighbours); else a contiguous run per wave
+    const uint32_t rel = a.cpw_interleave ? cj * gridDim.x * WAVES_PER_WG + lw : lw * a.cpw + cj;
+    if (rel >= a.nchunks) break;
+    const uint32_t c = a.chunk0 + rel;
+    const Chunk ch = a.chunks[c];
+    uint64_t bc[N];
+    #pragma unroll
+    for (int m = 0; m < N; ++m) bc[m] = a.bcoords[(uint64_t)ch.block * N + m];
+    const uint32_t run = (ch.len + S - 1) / S;
+    const uint32_t p0 = min((uint32_t)slot * run, ch.len), p1 = min(p0 + run, ch.len);
+    const uint32_t rounds = (run + L - 1) / L;  // the same for every slot
     for (int cb = 0; cb < (EXACT ? 1 : R); cb += C * L) {
         const int col = cb + sub * C;
         const uint32_t colb = (uint32_t)col * (uint32_t)sizeof(FT);
@@ -887,8 +914,19 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
             for (int cc = 0; cc < C; ++cc)
                 if (EXACT || col + cc < R) atomic_add_global(o + cc, acc[cc]);
         };
+        // a row piece from the factor, or (offset with bit 31 set) from its
+        // LDS window
         auto row_piece = [&](int m, uint32_t byte_off, AT (&x)[C]) {
-            if (EXACT || col + C <= R) {
+            if (byte_off >> 31) {
+                const unsigned char* t = s_tab + (byte_off & 0x7fffffffu);
+                if (EXACT || col + C <= R) {
+                    load_row<FT, C>(reinterpret_cast<const FT*>(t + colb), x);
+                } else {
+                    const FT* f = reinterpret_cast<const FT*>(t) + col;
+                    #pragma unroll
+                    for (int cc = 0; cc < C; ++cc) x[cc] = (col + cc < R) ? (AT)to_acc(f[cc]) : (AT)0;
+                }
+            } else if (EXACT || col + C <= R) {
                 load_row_buf<FT, C>(rs[m], byte_off + colb, x);
             } else {
                 const FT* f = reinterpret_cast<const FT*>(reinterpret_cast<const char*>(a.f[m]) + byte_off) + col;
@@ -914,11 +952,12 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
         for (int m = 0; m < N; ++m) carry[m] = 0;
         for (uint32_t r = 0; r < rounds; ++r) {
             const uint32_t base = p0 + r * L;
-            uint32_t off[N], row = 0;
+            uint32_t off[N], idx[N], row = 0;
             #pragma unroll
             for (int m = 0; m < N; ++m) {
                 const uint64_t i = ((coord >> a.pos[m]) & a.mask[m]) | bc[m];
-                off[m] = (uint32_t)i * (uint32_t)(R * (int)sizeof(FT));
+                idx[m] = (uint32_t)i;
+                off[m] = (uint32_t)i * row_bytes;
                 if (m == mode) row = (uint32_t)i;
             }
             uint32_t wv[RWP];
@@ -943,6 +982,16 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
                 #pragma unroll
                 for (int m = 0; m < N; ++m)
                     if (m == in[j]) wv[1 + j] = off[m];
+            }
+            // rows in a mode's LDS window: their offset there, bit 31 set
+            #pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                uint32_t ix = 0;
+                #pragma unroll
+                for (int m = 0; m < N; ++m)
+                    if (m == in[t]) ix = idx[m];
+                const uint32_t h = ix - a.hot_lo[t];
+                if (h < a.hot_n[t]) wv[1 + t] = 0x80000000u | (tab_off[t] + h * row_bytes);
             }
             __builtin_memcpy(&wv[N], &v, sizeof(AT));
             wave_lds_sync();  // the previous round's reads are done
@@ -1049,6 +1098,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
         const bool tail = slot == S - 1 || !next_have || next_cur != cur;
         if (have && tail) flush();
         wave_lds_sync();  // the next column block rewrites the records
+    }
     }
 }
 

@@ -112,6 +112,14 @@ int blco_dev_create_budget(const blco_host* h, int value_dtype, uint64_t device_
  * and the device bytes the tensor holds (blco_dev_bytes). */
 int blco_dev_residency(const blco_dev* d, uint64_t* resident_nnz, uint64_t* streamed_nnz,
                        uint64_t* device_bytes);
+/* Launch plan of kernel 14 (the row-ordered MTTKRP) for a mode after it has
+ * run there at this rank and factor dtype (no counterpart in the reference,
+ * whose kernels have no such plan): chunks of the copy per wave, whether a
+ * wave's chunks lie one grid width apart (1) or next to each other (0), and
+ * the hot factor rows the workgroups keep in LDS.  BLCO_EINVAL before the
+ * first such run. */
+int blco_dev_rows_plan(const blco_dev* d, int mode, int rank, int factor_dtype,
+                       uint32_t* chunks_per_wave, int* interleaved, uint64_t* lds_rows);
 /* delete_blcotensor_device, src/blco.cu:74-95 */
 void blco_dev_free(blco_dev* d);
 /* Bytes of device memory held by the tensor. */

@@ -148,6 +148,42 @@ def test_row_ordered_64bit_offsets(B, O, gpu, fibers, monkeypatch):
     dev.free()
 
 
+@pytest.fixture(scope="module")
+def big_zipf(B, O):
+    """40M unique nonzeros (over 32768 chunks of kernel 14's copies) with
+    Zipf(1.2) indices on the two long modes (their 128 most frequent rows
+    hold about half of the nonzeros)."""
+    dims = (300_000, 200_000, 3000)
+    t = B.Coo.generate(dims, 40_000_000, zipf=(1.2, 1.2, 0.0), gen_rank=4, value_kind=1, seed=36,
+                       topup=True)
+    assert t.nnz == 40_000_000
+    return dims, t, B.encode(t), O.Alto(dims, t.cidx, t.vals)
+
+
+@pytest.mark.parametrize("fibers,dtype", [("1", 1), ("0", 1), ("1", 0), ("0", 0)])
+def test_row_ordered_hot_windows(B, O, gpu, big_zipf, fibers, dtype, monkeypatch):
+    """Kernel 14 with its LDS windows of hot rows and several chunks per
+    wave (a grid-interleaved walk in the fiber form): every mode against the
+    oracle, FP32 at rank 32 and FP64 at rank 16 (128-byte rows), and the plan
+    shows the windows in use."""
+    monkeypatch.setenv("BLCO_FIBERS", fibers)
+    dims, t, h, at = big_zipf
+    R = 32 if dtype == 1 else 16
+    dev = B.DeviceTensor(h, value_dtype=dtype)
+    rng = np.random.default_rng(7)
+    U = [rng.random((d, R)) for d in dims]
+    windows = 0
+    for m in range(3):
+        print("hot windows: fibers", fibers, "dtype", dtype, "mode", m, flush=True)
+        e = rel(run_mttkrp(B, dev, U, m, dtype, B.K_SORTED), at.mttkrp(U, m))
+        assert e <= TOL[dtype], (m, e)
+        cpw, inter, rows = dev.rows_plan(m, R, dtype)
+        assert inter == (fibers == "1" and cpw > 1)
+        windows += rows > 0
+    assert windows >= 2
+    dev.free()
+
+
 @pytest.mark.parametrize("nprtn", [1, 4, 17])
 def test_lvl3_short_modes(B, O, gpu, nprtn):
     dims = (60, 90, 5000)
