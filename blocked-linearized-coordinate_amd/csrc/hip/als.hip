@@ -491,7 +491,8 @@ __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT*
                                                    double* __restrict__ stats)
 {
     constexpr int SLOTS = WG / G;
-    __shared__ double s_m[SLOTS][G + 1];
+    constexpr int RPS = 4;  // rows per slot per step: four loads in flight per lane
+    __shared__ double s_m[SLOTS * RPS][G + 1];
     __shared__ double s_st[SLOTS][G];
     const int j = threadIdx.x % G, slot = threadIdx.x / G;
     const bool col = j < R;
@@ -501,20 +502,28 @@ __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT*
     for (int k = 0; k < G; ++k)
         f[k] = (col && k < R) ? fac[j * R + k] * (colscale ? colscale[k] : 1.0) : 0.0;
     double st = which == 0 ? 0.0 : 1.0;
-    const uint64_t step = (uint64_t)gridDim.x * SLOTS;
+    const uint64_t step = (uint64_t)gridDim.x * SLOTS * RPS;
     // every lane runs the same number of iterations: the barriers below are uniform
     const uint64_t iters = (rows + step - 1) / step;
     for (uint64_t it = 0; it < iters; ++it) {
-        const uint64_t row = it * step + (uint64_t)blockIdx.x * SLOTS + slot;
-        const bool ok = row < rows && col;
-        s_m[slot][j] = ok ? (double)M[row * (uint64_t)R + j] : 0.0;
-        __syncthreads();
-        double t = 0.0;
+        // rows base + q * SLOTS + slot: each q a run of SLOTS consecutive rows
+        const uint64_t base = it * step + (uint64_t)blockIdx.x * SLOTS * RPS;
         #pragma unroll
-        for (int k = 0; k < G; ++k) t += f[k] * s_m[slot][k];
-        if (ok) {
-            U[row * (uint64_t)R + j] = from_f64<UT>(t);
-            st = which == 0 ? st + t * t : fmax(st, t);
+        for (int q = 0; q < RPS; ++q) {
+            const uint64_t row = base + (uint64_t)q * SLOTS + slot;
+            s_m[q * SLOTS + slot][j] = (row < rows && col) ? (double)M[row * (uint64_t)R + j] : 0.0;
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int q = 0; q < RPS; ++q) {
+            const uint64_t row = base + (uint64_t)q * SLOTS + slot;
+            double t = 0.0;
+            #pragma unroll
+            for (int k = 0; k < G; ++k) t += f[k] * s_m[q * SLOTS + slot][k];
+            if (row < rows && col) {
+                U[row * (uint64_t)R + j] = from_f64<UT>(t);
+                st = which == 0 ? st + t * t : fmax(st, t);
+            }
         }
         __syncthreads();
     }
@@ -524,6 +533,148 @@ __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT*
         double x = s_st[0][j];
         for (int q = 1; q < SLOTS; ++q) x = which == 0 ? x + s_st[q][j] : fmax(x, s_st[q][j]);
         stats[(uint64_t)blockIdx.x * R + j] = x;
+    }
+}
+
+// ------------------------------------------------------- fused solve + Gram
+
+// One pass of a mode's ALS update on MFMA: U = M diag(c) F^T row tile by row
+// tile (16 rows), then, from the same registers, the tile's contribution to
+// G = U^T U, the column statistics of U and (with ip) sum_i U_ic M_ic.
+//   solve  v_mfma_f64_16x16x4f64 with A[i][k] = M[r0 + i][4 ks + k] (lane l
+//          loads M[r0 + l%16][4 ks + l/16]) and B[k][j] = c_k F[16 ct + j][k]
+//          (held in registers for the whole launch); the result D[i][j] sits
+//          in register v of lane l as U[r0 + l/16 + 4 v][16 ct + l%16].
+//   Gram   that register v is, for the row step r0 + 4 v, exactly the A and B
+//          operand of the Gram's MFMA (lane l: U[r0 + 4v + l/16][16 t + l%16]),
+//          so the stored (rounded) values feed the Gram tiles directly: f64
+//          MFMA for f64 factors, f32 for f32 / bf16 ones, as in k_gram.
+// Rows past the end load as 0 and give 0 rows of U (not stored).  Each
+// workgroup's waves take tiles with a fixed stride; its partial Gram (and
+// ip) and column statistics are combined in LDS in wave order and written to
+// part[blockIdx] (R*R + R) and stats[blockIdx] (R): deterministic.
+template <typename MT, typename UT, int T>
+__global__ __launch_bounds__(WG) void k_solve_gram(const MT* __restrict__ M, UT* __restrict__ U, uint64_t rows,
+                                                   int R, const double* __restrict__ fac,
+                                                   const double* __restrict__ colscale, int which, bool want_ip,
+                                                   double* __restrict__ stats, double* __restrict__ part)
+{
+    constexpr bool F64 = std::is_same<UT, double>::value;
+    using AT = typename std::conditional<F64, double, float>::type;
+    using VT = typename std::conditional<F64, d4, f4>::type;
+    constexpr int NT = T * (T + 1) / 2;
+    constexpr int KS = 4 * T;  // k steps of 4 over the (padded) rank 16 T
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    // B operands: c_k F[16 ct + j][k] for k = 4 ks + k4, j = i16
+    double b[T][KS];
+    #pragma unroll
+    for (int ct = 0; ct < T; ++ct)
+        #pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + k4, j = 16 * ct + i16;
+            b[ct][ks] = (k < R && j < R) ? fac[j * R + k] * (colscale ? colscale[k] : 1.0) : 0.0;
+        }
+    VT g[NT];
+    #pragma unroll
+    for (int q = 0; q < NT; ++q) g[q] = VT{0, 0, 0, 0};
+    double st[T], ipa[T];
+    #pragma unroll
+    for (int ct = 0; ct < T; ++ct) {
+        st[ct] = which == 0 ? 0.0 : 1.0;
+        ipa[ct] = 0.0;
+    }
+    const uint64_t tiles = (rows + 15) / 16;
+    const uint64_t stride = (uint64_t)gridDim.x * WAVES_PER_WG;
+    for (uint64_t tile = (uint64_t)blockIdx.x * WAVES_PER_WG + w; tile < tiles; tile += stride) {
+        const uint64_t r0 = tile * 16;
+        d4 d[T];
+        #pragma unroll
+        for (int ct = 0; ct < T; ++ct) d[ct] = d4{0, 0, 0, 0};
+        const uint64_t ra = r0 + i16;
+        #pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + k4;
+            const double a = (ra < rows && k < R) ? (double)M[ra * (uint64_t)R + k] : 0.0;
+            #pragma unroll
+            for (int ct = 0; ct < T; ++ct) d[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct][ks], d[ct], 0, 0, 0);
+        }
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint64_t row = r0 + k4 + 4 * v;
+            AT x[T];
+            #pragma unroll
+            for (int ct = 0; ct < T; ++ct) {
+                const int col = 16 * ct + i16;
+                const double t = d[ct][v];
+                const bool ok = row < rows && col < R;
+                const UT u = from_f64<UT>(t);
+                if (ok) {
+                    U[row * (uint64_t)R + col] = u;
+                    st[ct] = which == 0 ? st[ct] + t * t : fmax(st[ct], t);
+                    if (want_ip) {
+                        const double mv = (double)M[row * (uint64_t)R + col];
+                        ipa[ct] += (double)(AT)as_f64(u) * mv;
+                    }
+                }
+                x[ct] = ok ? (AT)as_f64(u) : (AT)0;
+            }
+            int q = 0;
+            #pragma unroll
+            for (int ti = 0; ti < T; ++ti)
+                #pragma unroll
+                for (int tj = ti; tj < T; ++tj) {
+                    if constexpr (F64)
+                        g[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[ti], x[tj], g[q], 0, 0, 0);
+                    else
+                        g[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[ti], x[tj], g[q], 0, 0, 0);
+                    ++q;
+                }
+        }
+    }
+    // combine the waves: Gram tiles as in k_gram; statistics and ip over the
+    // four row groups (k4) of a wave, then over the waves, in order
+    __shared__ double s_g[WAVES_PER_WG][T * 16][T * 16 + 1];
+    __shared__ double s_v[WAVES_PER_WG][2][T * 16];
+    int q = 0;
+    #pragma unroll
+    for (int ti = 0; ti < T; ++ti)
+        #pragma unroll
+        for (int tj = ti; tj < T; ++tj) {
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = ti * 16 + (F64 ? k4 + 4 * v : 4 * k4 + v), j = tj * 16 + i16;
+                const double x = (double)g[q][v];
+                s_g[w][i][j] = x;
+                s_g[w][j][i] = x;
+            }
+            ++q;
+        }
+    #pragma unroll
+    for (int ct = 0; ct < T; ++ct) {
+        double a = st[ct], c = ipa[ct];
+        #pragma unroll
+        for (int o = 16; o < 64; o <<= 1) {
+            const double y = __shfl_xor(a, o);
+            a = which == 0 ? a + y : fmax(a, y);
+            c += __shfl_xor(c, o);
+        }
+        if (k4 == 0) {
+            s_v[w][0][ct * 16 + i16] = a;
+            s_v[w][1][ct * 16 + i16] = c;
+        }
+    }
+    __syncthreads();
+    double* out = part + (uint64_t)blockIdx.x * (uint64_t)(R * R + R);
+    for (int e = threadIdx.x; e < R * R; e += WG) {
+        const int i = e / R, j = e % R;
+        out[e] = ((s_g[0][i][j] + s_g[1][i][j]) + s_g[2][i][j]) + s_g[3][i][j];
+    }
+    for (int e = threadIdx.x; e < R; e += WG) {
+        out[R * R + e] = ((s_v[0][1][e] + s_v[1][1][e]) + s_v[2][1][e]) + s_v[3][1][e];
+        double a = s_v[0][0][e];
+        for (int x = 1; x < WAVES_PER_WG; ++x) a = which == 0 ? a + s_v[x][0][e] : fmax(a, s_v[x][0][e]);
+        stats[(uint64_t)blockIdx.x * R + e] = a;
     }
 }
 
@@ -770,8 +921,58 @@ int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, c
 // Workgroups of the row solve (each walks rows with a grid stride).
 uint32_t solve_parts(uint64_t rows)
 {
-    const uint64_t p = (rows + 31) / 32;
+    const uint64_t p = (rows + 127) / 128;
     return (uint32_t)(p < 1 ? 1 : p > 2048 ? 2048 : p);
+}
+
+uint32_t solve_gram_parts(uint64_t rows)
+{
+    const uint64_t p = (rows + 255) / 256;  // at least 4 row tiles per wave
+    return (uint32_t)(p < 1 ? 1 : p > GRAM_MAX_PARTS ? GRAM_MAX_PARTS : p);
+}
+
+template <typename MT, typename UT>
+static int solve_gram_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac, const double* cs, int which,
+                        bool ip, double* stats, double* part, uint32_t grid, hipStream_t st)
+{
+    switch ((R + 15) / 16) {
+        case 1: hipLaunchKernelGGL((k_solve_gram<MT, UT, 1>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, ip, stats, part); break;
+        case 2: hipLaunchKernelGGL((k_solve_gram<MT, UT, 2>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, ip, stats, part); break;
+        case 3: hipLaunchKernelGGL((k_solve_gram<MT, UT, 3>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, ip, stats, part); break;
+        case 4: hipLaunchKernelGGL((k_solve_gram<MT, UT, 4>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, ip, stats, part); break;
+        default: return fail(BLCO_EUNSUPPORTED, "rank %d > 64 in the fused solve", R);
+    }
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
+}
+
+int launch_solve_gram(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
+                      const double* colscale, int which, bool want_ip, double* stats, double* part, double* lambda,
+                      double* inv, double* dinv, double* G, double* ip, hipStream_t st)
+{
+    if (R < 1 || R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] in the fused solve", R);
+    if (rows == 0) return fail(BLCO_EINVAL, "fused solve of 0 rows");
+    const uint32_t grid = solve_gram_parts(rows);
+    int rc;
+    if (dtype == BLCO_F64)
+        rc = solve_gram_t<double, double>((const double*)M, (double*)U, rows, R, fac, colscale, which, want_ip, stats,
+                                          part, grid, st);
+    else if (dtype == BLCO_F32)
+        rc = solve_gram_t<float, float>((const float*)M, (float*)U, rows, R, fac, colscale, which, want_ip, stats, part,
+                                        grid, st);
+    else
+        rc = solve_gram_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, colscale, which, want_ip, stats, part,
+                                       grid, st);
+    if (!rc) rc = launch_lambda(stats, (int)grid, R, which, lambda, inv, st, dinv);
+    if (rc) return rc;
+    const int W = R * R + R;
+    int p2 = 0;
+    rc = fold_parts(part, (int)grid, W, 0, &p2, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gram_finish, dim3((W + WAVE - 1) / WAVE), dim3(WAVE), 0, st, part, p2, R, inv, colscale, G,
+                       want_ip ? ip : nullptr);
+    HIP_TRY(hipGetLastError());
+    return BLCO_OK;
 }
 
 int launch_pinv_general(double* A, int n, int* info, hipStream_t st)

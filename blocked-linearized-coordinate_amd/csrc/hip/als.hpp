@@ -42,6 +42,15 @@ uint32_t solve_parts(uint64_t rows);
 // be null: ones).  stats: solve_parts(rows) * R doubles.
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
                       int which, double* stats, hipStream_t st, const double* colscale = nullptr);
+// A mode's whole ALS update after its MTTKRP, in one pass over M and U on
+// MFMA: U = M diag(colscale) F^T (colscale may be null), lambda, inv and
+// dinv (may be null) from U's column statistics, G = diag(inv) U^T U
+// diag(inv), and with want_ip ip = inv o colscale o sum_i U_i M_i.  stats:
+// solve_gram_parts(rows) * R doubles; part: as for launch_gram.
+uint32_t solve_gram_parts(uint64_t rows);
+int launch_solve_gram(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
+                      const double* colscale, int which, bool want_ip, double* stats, double* part, double* lambda,
+                      double* inv, double* dinv, double* G, double* ip, hipStream_t st);
 // lambda, 1/lambda (and into dinv when non-null) from the row solve's
 // statistics (folded in place).
 int launch_lambda(double* stats, int nparts, int R, int which, double* lambda, double* inv, hipStream_t st,

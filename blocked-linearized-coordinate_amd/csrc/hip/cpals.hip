@@ -17,8 +17,10 @@
 // both sides; the MTTKRP of a later mode then carries the product c of the
 // other factors' lambdas in its columns, and that mode's solve multiplies
 // M by c^{-1} (folded into F).  Mathematically the same step; each mode's
-// update passes over U_n once (written by the solve, read by the Gram)
-// instead of rewriting it in place.  blco_cpals_get applies 1 / lambda.
+// update is one MFMA kernel that reads M once and writes U_n once, forming
+// the Gram from the solve's registers (launch_solve_gram), instead of a
+// solve, an in-place rescale and a Gram pass.  blco_cpals_get applies
+// 1 / lambda.
 //
 // Everything stays on one stream with no host round trip inside a sweep;
 // the host reads back the fit (8 bytes) once per iteration.  With use_graph
@@ -97,14 +99,8 @@ int enqueue_update(blco_cpals_state* s, int n, int which_norm, hipStream_t st)
     const int N = s->N, R = s->R;
     const uint64_t I = s->d->dims[n];
     const int last = s->order[N - 1];
-    int rc = launch_solve_rows(s->Mbuf, s->U[n], s->ft, I, R, s->fac, which_norm, s->stats, st, s->colscale);
-    if (!rc)
-        rc = launch_lambda(s->stats, (int)solve_parts(I), R, which_norm, s->lambda, s->inv, st,
-                           s->dinv + (size_t)n * R);
-    if (!rc)
-        rc = launch_gram_of_scaled(s->U[n], s->ft, I, R, s->inv, n == last ? s->Mbuf : nullptr, s->colscale,
-                                   s->gram_part, s->G[n], s->ip, st);
-    return rc;
+    return launch_solve_gram(s->Mbuf, s->U[n], s->ft, I, R, s->fac, s->colscale, which_norm, n == last, s->stats,
+                             s->gram_part, s->lambda, s->inv, s->dinv + (size_t)n * R, s->G[n], s->ip, st);
 }
 
 int enqueue_fit(blco_cpals_state* s, hipStream_t st)
