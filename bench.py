@@ -272,6 +272,7 @@ def run_large(B, args):
         log("%s: %.4f s/iter (host %.4f), first iteration %.1f s, MTTKRP ms/mode %s, kernels %s, fit %.9f"
             % (name, r["s_per_iter"], r["wall"] / max(args.steps, 1), r["first_iter_s"],
                np.round(r["per_mode_ms"], 2).tolist(), r["kernels"].tolist(), r["fits"][-1]))
+    sel = selector_vs_rule(B, dev, U0, R, args, dims, nnz) if "fp32" in res else None
     errs = None
     if out64 is not None and "fp32" in res:
         out32 = mttkrp_outputs(B, dev, U0, R, B.F32)
@@ -349,9 +350,29 @@ def run_large(B, args):
         out["fit_bf16"] = float(res["bf16"]["fits"][-1])
     if streamed:
         out["fp32_streamed"] = streamed
+    if sel is not None:
+        out["selector"] = sel
     if cpu is not None:
         out["cpu_baseline"] = cpu_baseline_json(cpu, sample_desc)
     print(json.dumps(out), flush=True)
+
+
+def selector_vs_rule(B, dev, U0, R, args, dims, nnz):
+    """The learned selector's choice per mode (kernel id, lvl3 partitions,
+    tiles per wave) beside the -k 10 rule's; when they differ, the FP32 form
+    is also timed under the rule."""
+    if not B.selector_installed():
+        return None
+    choice = [list(dev.select_kernel(m, R, B.F32)) for m in range(len(dims))]
+    rule = [[dev.auto_kernel(m, R), 0, 0] for m in range(len(dims))]
+    out = {"choice_per_mode": choice, "rule_per_mode": rule}
+    if any(c[0] != r[0] or c[2] != 0 or (c[0] == B.K_LVL3 and c[1] != 0) for c, r in zip(choice, rule)):
+        r = summarize(run_form(B, dev, U0, R, B.F32, B.K_AUTO, args.warmup, args.steps), dims, nnz, R, 4, 4)
+        out["s_per_iter_fp32_rule"] = round(r["s_per_iter"], 6)
+        out["mttkrp_ms_per_mode_fp32_rule"] = [round(float(x), 3) for x in r["per_mode_ms"]]
+        log("fp32 under the -k 10 rule: %.4f s/iter, MTTKRP ms/mode %s"
+            % (r["s_per_iter"], np.round(r["per_mode_ms"], 2).tolist()))
+    return out
 
 
 def batch_specs():
@@ -440,6 +461,7 @@ def run_workload(B, args):
     results = {}
     forms = [("fp32", B.F32, 1)] + ([] if args.no_fp64 else [("fp64", B.F64, 0)])
     errs = None
+    sel = None
     for name, prec, vdt in forms:
         dev = B.DeviceTensor(h, value_dtype=vdt)
         kernel = B.K_SELECT if prec != B.F64 else B.K_AUTO
@@ -453,6 +475,8 @@ def run_workload(B, args):
                               nprtn=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())))
             errs = [rel_err(out32[m], quiet_stdout(at.mttkrp, U0, m)) for m in range(len(dims))]
             log("fp32 MTTKRP relative error per mode vs the FP64 oracle: %s" % ["%.2e" % e for e in errs])
+        if name == "fp32":
+            sel = selector_vs_rule(B, dev, U0, R, args, dims, nnz)
         dev.free()
         results[name] = r
         log("%s: %.4f s/iter (host %.4f), MTTKRP ms/mode %s, GB/s %s, kernels %s, final fit %.9f" % (
@@ -490,6 +514,8 @@ def run_workload(B, args):
                     "mttkrp_ms_per_mode_fp64": [round(float(x), 4) for x in r64["per_mode_ms"]],
                     "fit_fp64": float(r64["fits"][-1]),
                     "fit_diff_fp32_vs_fp64": float(abs(m["fits"][-1] - r64["fits"][-1]))})
+    if sel is not None:
+        out["selector"] = sel
     if cpu is not None:
         out["cpu_baseline"] = cpu_baseline_json(
             cpu, "oracle ALTO CPU CP-ALS (OpenMP) on the full tensor, rank %d FP64, mean of %d iterations "

@@ -23,7 +23,7 @@ F64, F32, BF16 = 0, 1, 2
 K_LVL1, K_LVL2, K_LVL3, K_AUTO, K_BATCHED, K_BATCHED2, K_ATOMIC, K_SORTED, K_SELECT = \
     1, 2, 3, 10, 11, 12, 13, 14, 20
 MAX_MODES = 5
-SEL_NFEAT, SEL_NCAND = 24, 11
+SEL_NFEAT, SEL_NCAND = 24, 15
 
 _lib = None
 
@@ -135,7 +135,7 @@ def lib():
             "blco_cpals": (C.c_int, [vp, C.POINTER(CpalsOpts), C.POINTER(vp), f64p, f64p,
                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
             "blco_mttkrp_time": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), vp, C.c_int,
-                                           C.c_int, C.c_int, C.POINTER(C.c_float),
+                                           C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
                                            C.POINTER(C.c_int)]),
             "blco_mlp_create": (C.c_int, [C.c_int, i32p, C.c_uint64, C.POINTER(vp)]),
             "blco_mlp_free": (None, [vp]),
@@ -147,14 +147,15 @@ def lib():
             "blco_mlp_adam_step": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float]),
             "blco_mlp_train": (C.c_int, [vp, f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float,
                                          C.c_uint64, vp]),
-            "blco_sel_candidate": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "blco_sel_candidate": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]),
             "blco_sel_valid": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
             "blco_mode_features": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, f32p]),
             "blco_selector_install": (C.c_int, [C.c_int, i32p, f32p, f32p, f32p]),
             "blco_selector_load": (C.c_int, [C.c_char_p]),
             "blco_selector_installed": (C.c_int, []),
             "blco_select_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
-                                             C.POINTER(C.c_int)]),
+                                             C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "blco_batch_create": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
             "blco_batch_free": (None, [vp]),
             "blco_batch_tune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(vp),
@@ -469,6 +470,7 @@ class DeviceTensor:
         self.nmodes = host.nmodes
         self.dims = host.dims.copy()
         self.nnz = host.nnz
+        self.nblocks = host.nblocks
         self.value_dtype = value_dtype
 
     @classmethod
@@ -547,10 +549,11 @@ class DeviceTensor:
         return bool(lib().blco_sel_valid(self.h, mode, rank, factor_dtype, c))
 
     def select_kernel(self, mode, rank, factor_dtype=F32):
-        """(kernel id, lvl3 partitions) that kernel id 20 runs for this mode."""
-        k, n = C.c_int(0), C.c_int(0)
-        _check(lib().blco_select_kernel(self.h, mode, rank, factor_dtype, C.byref(k), C.byref(n)))
-        return k.value, n.value
+        """(kernel id, lvl3 partitions, tiles per wave) that kernel id 20 runs
+        for this mode."""
+        k, n, t = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(lib().blco_select_kernel(self.h, mode, rank, factor_dtype, C.byref(k), C.byref(n), C.byref(t)))
+        return k.value, n.value, t.value
 
     def auto_kernel(self, mode, rank):
         return lib().blco_auto_kernel(self.h, mode, rank)
@@ -572,21 +575,22 @@ def mttkrp(dev, mode, rank, factors, out, factor_dtype=F64, kernel=K_AUTO, nprtn
                              thread_cf, _stream(stream)))
 
 
-def mttkrp_time(dev, mode, rank, factors, out, factor_dtype=F64, kernel=K_AUTO, nprtn=0, reps=3):
+def mttkrp_time(dev, mode, rank, factors, out, factor_dtype=F64, kernel=K_AUTO, nprtn=0, reps=3, thread_cf=0):
     """Mean event-timed ms of one MTTKRP launch sequence (after one untimed
     call), and the kernel id that ran."""
     ptrs = _ptr_array([_dptr(f) for f in factors])
     ms, used = C.c_float(0), C.c_int(0)
     _check(lib().blco_mttkrp_time(dev.h, mode, rank, factor_dtype, ptrs, _dptr(out), kernel, nprtn,
-                                  reps, C.byref(ms), C.byref(used)))
+                                  thread_cf, reps, C.byref(ms), C.byref(used)))
     return ms.value, used.value
 
 
 def sel_candidate(c):
-    """Candidate c of the selector -> (kernel id, lvl3 partitions)."""
-    k, n = C.c_int(0), C.c_int(0)
-    _check(lib().blco_sel_candidate(c, C.byref(k), C.byref(n)))
-    return k.value, n.value
+    """Candidate c of the selector -> (kernel id, lvl3 partitions, tiles per
+    wave; 0 = the kernel's default)."""
+    k, n, t = C.c_int(0), C.c_int(0), C.c_int(0)
+    _check(lib().blco_sel_candidate(c, C.byref(k), C.byref(n), C.byref(t)))
+    return k.value, n.value, t.value
 
 
 def selector_install(widths, params, xmean, xstd):

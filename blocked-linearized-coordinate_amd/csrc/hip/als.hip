@@ -10,8 +10,7 @@
 //              f32 form leaves D[4*(l/16) + v][l%16] in register v of lane l,
 //              the f64 form D[l/16 + 4*v][l%16] (checked on the hardware).  Per-workgroup partial Grams are
 //              summed in a fixed order (deterministic).  Optionally the same
-//              pass scales U by 1/lambda first (normalisation) and forms the
-//              fit inner products sum_i U_ir M_ir.
+//              pass forms the fit inner products sum_i U_ir M_ir.
 //   factorize  one workgroup: V = prod_{m != n} G_m (Hadamard), lower
 //              Cholesky in LAPACK dpotf2 order in LDS; when a pivot falls to
 //              R * eps * max V_ii or below, cyclic Jacobi eigendecomposition
@@ -21,12 +20,14 @@
 //              in (f64, or f32 for the f32 / bf16 factor forms).
 //              On success the explicit inverse V^{-1} = L^{-T} L^{-1} is
 //              formed (so both outcomes leave one R x R matrix F).
-//   solve rows one thread per row: u = F m with F broadcast from LDS; the
-//              same pass forms per-workgroup column statistics for the norm
-//              (sum of squares, or max).
-//   lambda     one workgroup: lambda_r = sqrt(sum) (L2) or max(1, max) (max
-//              norm, src/kruskal_model.cpp:183-223), and 1/lambda (0 for a
-//              zero column).
+//   solve rows a group of R lanes per row: u = F m with the row of M staged
+//              in LDS; the same pass forms per-workgroup column statistics
+//              for the norm (sum of squares, or max).
+//   solve+gram the CP-ALS update of a mode in one pass on MFMA (k_solve_gram):
+//              the solve's result registers feed the Gram tiles directly.
+//   lambda     lambda_r = sqrt(sum) (L2) or max(1, max) (max norm,
+//              src/kruskal_model.cpp:183-223), and 1/lambda (0 for a zero
+//              column), from the statistics folded in two passes.
 //   fit        one workgroup: fit = 1 - r / ||X|| with
 //              r^2 = ||X||^2 + |sum_ij l_i l_j prod_n G_n(i,j)| - 2 sum_r l_r ip_r
 //              (include/cpd.hpp:190-272).
@@ -49,9 +50,8 @@ template <typename T> __device__ __forceinline__ bool is_zero(T x) { return x ==
 // Partial Gram (and optional scaling / inner products) of rows
 // [g * rows_per_wg, ...) for workgroup g.  T tiles of 16 columns.
 template <typename UT, int T>
-__global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, int R,
+__global__ __launch_bounds__(WG) void k_gram(const UT* __restrict__ U, uint64_t rows, int R,
                                              uint64_t rows_per_wg,
-                                             const double* __restrict__ inv_scale,
                                              const void* __restrict__ M,
                                              double* __restrict__ part)
 {
@@ -69,12 +69,6 @@ __global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, 
     double ip[T];
     #pragma unroll
     for (int t = 0; t < T; ++t) ip[t] = 0.0;
-    double sc[T];
-    #pragma unroll
-    for (int t = 0; t < T; ++t) {
-        int col = t * 16 + i16;
-        sc[t] = (inv_scale && col < R) ? inv_scale[col] : 1.0;
-    }
     for (uint64_t r0 = rb + (uint64_t)w * 4; r0 < re; r0 += WAVES_PER_WG * 4) {
         const uint64_t row = r0 + k4;
         AT a[T];
@@ -83,14 +77,7 @@ __global__ __launch_bounds__(WG) void k_gram(UT* __restrict__ U, uint64_t rows, 
             const int col = t * 16 + i16;
             AT x = 0;
             if (row < re && col < R) {
-                UT* p = U + row * (uint64_t)R + col;
-                if (inv_scale) {
-                    UT s = from_f64<UT>(as_f64(*p) * sc[t]);
-                    *p = s;
-                    x = (AT)as_f64(s);
-                } else {
-                    x = (AT)as_f64(*p);
-                }
+                x = (AT)as_f64(U[row * (uint64_t)R + col]);
                 if (M) {
                     double mv = F64 ? reinterpret_cast<const double*>(M)[row * (uint64_t)R + col]
                                     : (double)reinterpret_cast<const float*>(M)[row * (uint64_t)R + col];
@@ -486,8 +473,7 @@ __global__ __launch_bounds__(WG) void k_pinv_general(double* __restrict__ A, int
 template <typename MT, typename UT, int G>
 __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT* __restrict__ U,
                                                    uint64_t rows, int R,
-                                                   const double* __restrict__ fac,
-                                                   const double* __restrict__ colscale, int which,
+                                                   const double* __restrict__ fac, int which,
                                                    double* __restrict__ stats)
 {
     constexpr int SLOTS = WG / G;
@@ -496,11 +482,9 @@ __global__ __launch_bounds__(WG) void k_solve_rows(const MT* __restrict__ M, UT*
     __shared__ double s_st[SLOTS][G];
     const int j = threadIdx.x % G, slot = threadIdx.x / G;
     const bool col = j < R;
-    // row j of F diag(c): the MTTKRP result's columns carry c's inverse
     double f[G];
     #pragma unroll
-    for (int k = 0; k < G; ++k)
-        f[k] = (col && k < R) ? fac[j * R + k] * (colscale ? colscale[k] : 1.0) : 0.0;
+    for (int k = 0; k < G; ++k) f[k] = (col && k < R) ? fac[j * R + k] : 0.0;
     double st = which == 0 ? 0.0 : 1.0;
     const uint64_t step = (uint64_t)gridDim.x * SLOTS * RPS;
     // every lane runs the same number of iterations: the barriers below are uniform
@@ -807,15 +791,15 @@ __global__ void k_vecsum(double* __restrict__ x, uint64_t n)
 // ------------------------------------------------------------ launchers
 
 template <typename UT>
-static int gram_t(UT* U, uint64_t rows, int R, const double* inv, const void* M, double* part,
+static int gram_t(const UT* U, uint64_t rows, int R, const void* M, double* part,
                   int nparts, uint64_t rpw, hipStream_t st)
 {
     int T = (R + 15) / 16;
     switch (T) {
-        case 1: hipLaunchKernelGGL((k_gram<UT, 1>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
-        case 2: hipLaunchKernelGGL((k_gram<UT, 2>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
-        case 3: hipLaunchKernelGGL((k_gram<UT, 3>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
-        case 4: hipLaunchKernelGGL((k_gram<UT, 4>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, inv, M, part); break;
+        case 1: hipLaunchKernelGGL((k_gram<UT, 1>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, M, part); break;
+        case 2: hipLaunchKernelGGL((k_gram<UT, 2>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, M, part); break;
+        case 3: hipLaunchKernelGGL((k_gram<UT, 3>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, M, part); break;
+        case 4: hipLaunchKernelGGL((k_gram<UT, 4>), dim3(nparts), dim3(WG), 0, st, U, rows, R, rpw, M, part); break;
         default: return fail(BLCO_EUNSUPPORTED, "rank %d > 64 not supported by the Gram kernel", R);
     }
     HIP_TRY(hipGetLastError());
@@ -837,39 +821,26 @@ int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg)
     return BLCO_OK;
 }
 
-static int gram_any(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
-                    double* part, double* G, double* ip, const double* post_inv, const double* colscale,
-                    hipStream_t st)
+int launch_gram(const void* U, int dtype, uint64_t rows, int R, const void* M,
+                double* part, double* G, double* ip, hipStream_t st)
 {
     if (R < 1 || R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d outside [1, 64] for the Gram kernel", R);
     int nparts;
     uint64_t rpw;
     gram_parts(rows, &nparts, &rpw);
     int rc;
-    if (dtype == BLCO_F64) rc = gram_t<double>((double*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
-    else if (dtype == BLCO_F32) rc = gram_t<float>((float*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
-    else rc = gram_t<bf16>((bf16*)U, rows, R, inv_scale, M, part, nparts, rpw, st);
+    if (dtype == BLCO_F64) rc = gram_t<double>((const double*)U, rows, R, M, part, nparts, rpw, st);
+    else if (dtype == BLCO_F32) rc = gram_t<float>((const float*)U, rows, R, M, part, nparts, rpw, st);
+    else rc = gram_t<bf16>((const bf16*)U, rows, R, M, part, nparts, rpw, st);
     if (rc) return rc;
     const int W = R * R + R;
     int p2 = 0;
     rc = fold_parts(part, nparts, W, 0, &p2, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gram_finish, dim3((W + WAVE - 1) / WAVE), dim3(WAVE), 0, st, part, p2, R, post_inv,
-                       colscale, G, ip);
+    hipLaunchKernelGGL(k_gram_finish, dim3((W + WAVE - 1) / WAVE), dim3(WAVE), 0, st, part, p2, R, nullptr, nullptr,
+                       G, ip);
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
-}
-
-int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
-                double* part, double* G, double* ip, hipStream_t st)
-{
-    return gram_any(U, dtype, rows, R, inv_scale, M, part, G, ip, nullptr, nullptr, st);
-}
-
-int launch_gram_of_scaled(const void* U, int dtype, uint64_t rows, int R, const double* inv, const void* M,
-                          const double* colscale, double* part, double* G, double* ip, hipStream_t st)
-{
-    return gram_any(const_cast<void*>(U), dtype, rows, R, nullptr, M, part, G, ip, inv, colscale, st);
 }
 
 double gram_eps(int factor_dtype)
@@ -891,7 +862,7 @@ int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps,
 }
 
 template <typename MT, typename UT>
-static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac, const double* cs,
+static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* fac,
                         int which, double* stats, hipStream_t st)
 {
     if (rows == 0) return BLCO_OK;
@@ -899,23 +870,21 @@ static int solve_rows_t(const MT* M, UT* U, uint64_t rows, int R, const double* 
     if (R > 64) return fail(BLCO_EUNSUPPORTED, "rank %d > 64 in the row solve", R);
     const int g = R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64;
     switch (g) {
-        case 8: hipLaunchKernelGGL((k_solve_rows<MT, UT, 8>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
-        case 16: hipLaunchKernelGGL((k_solve_rows<MT, UT, 16>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
-        case 32: hipLaunchKernelGGL((k_solve_rows<MT, UT, 32>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
-        default: hipLaunchKernelGGL((k_solve_rows<MT, UT, 64>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, cs, which, stats); break;
+        case 8: hipLaunchKernelGGL((k_solve_rows<MT, UT, 8>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        case 16: hipLaunchKernelGGL((k_solve_rows<MT, UT, 16>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        case 32: hipLaunchKernelGGL((k_solve_rows<MT, UT, 32>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
+        default: hipLaunchKernelGGL((k_solve_rows<MT, UT, 64>), dim3(grid), dim3(WG), 0, st, M, U, rows, R, fac, which, stats); break;
     }
     HIP_TRY(hipGetLastError());
     return BLCO_OK;
 }
 
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
-                      int which, double* stats, hipStream_t st, const double* colscale)
+                      int which, double* stats, hipStream_t st)
 {
-    if (dtype == BLCO_F64)
-        return solve_rows_t<double, double>((const double*)M, (double*)U, rows, R, fac, colscale, which, stats, st);
-    if (dtype == BLCO_F32)
-        return solve_rows_t<float, float>((const float*)M, (float*)U, rows, R, fac, colscale, which, stats, st);
-    return solve_rows_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, colscale, which, stats, st);
+    if (dtype == BLCO_F64) return solve_rows_t<double, double>((const double*)M, (double*)U, rows, R, fac, which, stats, st);
+    if (dtype == BLCO_F32) return solve_rows_t<float, float>((const float*)M, (float*)U, rows, R, fac, which, stats, st);
+    return solve_rows_t<float, bf16>((const float*)M, (bf16*)U, rows, R, fac, which, stats, st);
 }
 
 // Workgroups of the row solve (each walks rows with a grid stride).

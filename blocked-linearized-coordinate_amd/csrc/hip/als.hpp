@@ -15,16 +15,10 @@ struct GramSet {
 
 constexpr int GRAM_MAX_PARTS = 2048;  // partial Gram blocks (workgroups) at most
 int gram_parts(uint64_t rows, int* nparts, uint64_t* rows_per_wg);
-// G = U^T U (f64, R x R) from dtype U; with inv_scale, U is first scaled in
-// place by inv_scale[col]; with M, ip[col] = sum_i U_ic M_ic.  part needs
-// gram_parts(rows) * (R*R + R) doubles.
-int launch_gram(void* U, int dtype, uint64_t rows, int R, const double* inv_scale, const void* M,
+// G = U^T U (f64, R x R) from dtype U; with M, ip[col] = sum_i U_ic M_ic.
+// part needs gram_parts(rows) * (R*R + R) doubles.
+int launch_gram(const void* U, int dtype, uint64_t rows, int R, const void* M,
                 double* part, double* G, double* ip, hipStream_t st);
-// G and ip of the factor U diag(inv) without writing it: G = diag(inv) U^T U
-// diag(inv), ip[col] = inv[col] colscale[col] sum_i U_ic M_ic (colscale may
-// be null: ones).  part as for launch_gram.
-int launch_gram_of_scaled(const void* U, int dtype, uint64_t rows, int R, const double* inv, const void* M,
-                          const double* colscale, double* part, double* G, double* ip, hipStream_t st);
 // Machine epsilon of the precision the Grams of factor_dtype factors are
 // accumulated in (f64 for f64 factors, f32 for f32 and bf16 ones).
 double gram_eps(int factor_dtype);
@@ -38,10 +32,9 @@ int launch_factorize(const GramSet& gs, int nmodes, int mode, int R, double eps,
 // (device, may be null): 0 converged, else the sweep count.
 int launch_pinv_general(double* A, int n, int* info, hipStream_t st);
 uint32_t solve_parts(uint64_t rows);
-// U = diag-scaled solve: row i of U = (c o M_i) F^T with c = colscale (may
-// be null: ones).  stats: solve_parts(rows) * R doubles.
+// U = M F^T row by row; stats: solve_parts(rows) * R doubles.
 int launch_solve_rows(const void* M, void* U, int dtype, uint64_t rows, int R, const double* fac,
-                      int which, double* stats, hipStream_t st, const double* colscale = nullptr);
+                      int which, double* stats, hipStream_t st);
 // A mode's whole ALS update after its MTTKRP, in one pass over M and U on
 // MFMA: U = M diag(colscale) F^T (colscale may be null), lambda, inv and
 // dinv (may be null) from U's column statistics, G = diag(inv) U^T U

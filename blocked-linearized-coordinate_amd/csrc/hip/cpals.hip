@@ -269,7 +269,7 @@ int blco_cpals_create(blco_dev* d, const blco_cpals_opts* o, const double* const
         s->normX2 = acc;
     }
     for (int m = 0; m < N && !rc; ++m)
-        rc = launch_gram(s->U[m], s->ft, d->dims[m], R, nullptr, nullptr, s->gram_part, s->G[m], nullptr, s->st);
+        rc = launch_gram(s->U[m], s->ft, d->dims[m], R, nullptr, s->gram_part, s->G[m], nullptr, s->st);
     if (rc) return bad(rc);
     if (hipStreamSynchronize(s->st) != hipSuccess) return bad(fail(BLCO_EHIP, "setup failed"));
     // build the chunk table outside any capture

@@ -832,7 +832,8 @@ int mttkrp_prepare(blco_dev* d, int mode, int rank, int ftype, int kernel)
 {
     if (!d || mode < 0 || mode >= d->nmodes || !d->nnz || d->stream_data) return BLCO_OK;
     int np = 0;
-    if (kernel == BLCO_K_SELECT && !select_kernel(d, mode, rank, ftype, &kernel, &np)) kernel = BLCO_K_AUTO;
+    int stpw = 0;
+    if (kernel == BLCO_K_SELECT && !select_kernel(d, mode, rank, ftype, &kernel, &np, &stpw)) kernel = BLCO_K_AUTO;
     const bool chosen = kernel == BLCO_K_AUTO;
     if (chosen) kernel = auto_kernel(d, mode, rank, ftype);
     if (kernel != BLCO_K_SORTED) return BLCO_OK;
@@ -865,10 +866,11 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
     if (nprtn > 1024) return fail(BLCO_EINVAL, "nprtn %d > 1024", nprtn);
     if (!st) st = d->stream;
     if (kernel == BLCO_K_SELECT) {
-        int k = 0, np = 0;
-        if (select_kernel(d, mode, rank, ftype, &k, &np)) {
+        int k = 0, np = 0, stpw = 0;
+        if (select_kernel(d, mode, rank, ftype, &k, &np, &stpw)) {
             kernel = k;
             if (k == BLCO_K_LVL3) nprtn = np;
+            if (tpw <= 0) tpw = stpw;  // the caller's thread_cf wins
         } else {
             kernel = BLCO_K_AUTO;
         }
@@ -1715,11 +1717,11 @@ int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype, void* const* 
 }
 
 int blco_mttkrp_time(blco_dev* d, int mode, int rank, int factor_dtype, void* const* factors, void* out,
-                     int kernel, int nprtn, int reps, float* ms, int* used_kernel)
+                     int kernel, int nprtn, int thread_cf, int reps, float* ms, int* used_kernel)
 {
     if (!ms || reps < 1) return fail(BLCO_EINVAL, "bad timing arguments");
     // one untimed call builds the tables and the scratch
-    int rc = mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, 0, nullptr, used_kernel);
+    int rc = mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, thread_cf, nullptr, used_kernel);
     if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -1729,7 +1731,7 @@ int blco_mttkrp_time(blco_dev* d, int mode, int rank, int factor_dtype, void* co
     }
     hipError_t e = hipEventRecord(e0, d->stream);
     for (int r = 0; r < reps && !rc && e == hipSuccess; ++r)
-        rc = mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, 0, nullptr, nullptr);
+        rc = mttkrp(d, mode, rank, factor_dtype, factors, out, kernel, nprtn, thread_cf, nullptr, nullptr);
     if (e == hipSuccess) e = hipEventRecord(e1, d->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float t = 0.f;
@@ -1759,7 +1761,7 @@ int blco_gram(const void* U, int dtype, uint64_t rows, int rank, double* G, void
     gram_parts(rows, &nparts, &rpw);
     double* part = nullptr;
     HIP_TRY(hipMalloc(&part, sizeof(double) * (size_t)nparts * (rank * rank + rank)));
-    int rc = launch_gram(const_cast<void*>(U), dtype, rows, rank, nullptr, nullptr, part, G, nullptr, st);
+    int rc = launch_gram(U, dtype, rows, rank, nullptr, part, G, nullptr, st);
     hipError_t e = hipStreamSynchronize(st);
     (void)hipFree(part);
     if (rc) return rc;
@@ -1871,7 +1873,7 @@ int blco_fit(double* const* grams, int nmodes, int rank, const double* lambda, c
     HIP_TRY(hipMalloc(&G, sizeof(double) * rank * rank));
     HIP_TRY(hipMalloc(&ip, sizeof(double) * rank));
     HIP_TRY(hipMalloc(&out, sizeof(double) * 4));
-    int rc = launch_gram(const_cast<void*>(U_last), dtype, rows, rank, nullptr, M_last, part, G, ip, st);
+    int rc = launch_gram(U_last, dtype, rows, rank, M_last, part, G, ip, st);
     if (!rc) rc = launch_fit(gs, nmodes, rank, lambda, ip, normX2, out, st);
     double h[3] = {0, 0, 0};
     hipError_t e = hipMemcpyAsync(h, out, sizeof(double) * 3, hipMemcpyDeviceToHost, st);

@@ -50,7 +50,7 @@ struct WindowPlan {
 
 // A selector decision for (mode, rank, factor type); kernel 0 = none.
 struct SelCache {
-    int mode, rank, ftype, kernel, nprtn;
+    int mode, rank, ftype, kernel, nprtn, tpw;
     uint64_t gen;  // selector generation the decision was made with
 };
 
@@ -162,7 +162,7 @@ const WindowPlan* window_plan(blco_dev* d, int mode, int rank, int acc_bytes);
 int auto_kernel(const blco_dev* d, int mode, int rank, int ftype);
 bool lvl3_output_fits(const blco_dev* d, int mode, int rank, int ftype);
 int mode_features(const blco_dev* d, int mode, int rank, int ftype, float* out);
-bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int* nprtn);
+bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int* nprtn, int* tpw);
 int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, void* out, int kernel,
            int nprtn, int tpw, hipStream_t st, int* used_kernel);
 // The host-side and device-side set-up that mttkrp() of (mode, rank, ftype,

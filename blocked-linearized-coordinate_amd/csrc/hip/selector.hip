@@ -39,11 +39,14 @@ namespace blco {
 
 namespace {
 
-// candidate c -> (kernel id, lvl3 partition count)
+// candidate c -> (kernel id, lvl3 partition count, tiles of 64 nonzeros per
+// wave: 0 for the kernel's default)
 constexpr int CAND_KERNEL[BLCO_SEL_NCAND] = {BLCO_K_BATCHED2, BLCO_K_BATCHED, BLCO_K_ATOMIC, BLCO_K_LVL3,
                                              BLCO_K_LVL3,     BLCO_K_LVL3,    BLCO_K_LVL3,  BLCO_K_LVL3,
-                                             BLCO_K_LVL3,     BLCO_K_LVL3,    BLCO_K_SORTED};
-constexpr int CAND_NPRTN[BLCO_SEL_NCAND] = {1, 1, 1, 1, 2, 4, 8, 16, 32, 64, 1};
+                                             BLCO_K_LVL3,     BLCO_K_LVL3,    BLCO_K_SORTED, BLCO_K_BATCHED,
+                                             BLCO_K_BATCHED,  BLCO_K_SORTED,  BLCO_K_SORTED};
+constexpr int CAND_NPRTN[BLCO_SEL_NCAND] = {1, 1, 1, 1, 2, 4, 8, 16, 32, 64, 1, 1, 1, 1, 1};
+constexpr int CAND_TPW[BLCO_SEL_NCAND] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 16, 8, 32};
 
 struct Selector {
     std::mutex mu;
@@ -174,7 +177,7 @@ bool candidate_valid(const blco_dev* d, int mode, int rank, int ftype, int c)
 
 // The learned choice for (mode, rank, ftype); false when no network is
 // installed.
-bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int* nprtn)
+bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int* nprtn, int* tpw)
 {
     Selector& s = sel();
     const uint64_t gen = s.gen.load();
@@ -182,11 +185,12 @@ bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int*
         if (c.mode == mode && c.rank == rank && c.ftype == ftype && c.gen == gen) {
             *kernel = c.kernel;
             *nprtn = c.nprtn;
+            *tpw = c.tpw;
             return c.kernel != 0;
         }
     std::lock_guard<std::mutex> lk(s.mu);
     try_default(s);
-    SelCache entry{mode, rank, ftype, 0, 1, s.gen.load()};
+    SelCache entry{mode, rank, ftype, 0, 1, 0, s.gen.load()};
     if (s.nlayers > 0) {
         int dev = 0;
         (void)hipGetDevice(&dev);
@@ -211,6 +215,7 @@ bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int*
             if (best >= 0) {
                 entry.kernel = CAND_KERNEL[best];
                 entry.nprtn = CAND_NPRTN[best];
+                entry.tpw = CAND_TPW[best];
             }
         }
     }
@@ -219,11 +224,13 @@ bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int*
             c = entry;
             *kernel = entry.kernel;
             *nprtn = entry.nprtn;
+            *tpw = entry.tpw;
             return entry.kernel != 0;
         }
     d->sel_cache.push_back(entry);
     *kernel = entry.kernel;
     *nprtn = entry.nprtn;
+    *tpw = entry.tpw;
     return entry.kernel != 0;
 }
 
@@ -233,12 +240,13 @@ using namespace blco;
 
 extern "C" {
 
-int blco_sel_candidate(int c, int* kernel, int* nprtn)
+int blco_sel_candidate(int c, int* kernel, int* nprtn, int* thread_cf)
 {
-    if (c < 0 || c >= BLCO_SEL_NCAND || !kernel || !nprtn)
+    if (c < 0 || c >= BLCO_SEL_NCAND || !kernel || !nprtn || !thread_cf)
         return fail(BLCO_EINVAL, "candidate %d outside 0..%d", c, BLCO_SEL_NCAND - 1);
     *kernel = CAND_KERNEL[c];
     *nprtn = CAND_NPRTN[c];
+    *thread_cf = CAND_TPW[c];
     return BLCO_OK;
 }
 
@@ -284,14 +292,15 @@ int blco_selector_installed(void)
     return s.nlayers > 0 ? 1 : 0;
 }
 
-int blco_select_kernel(blco_dev* d, int mode, int rank, int factor_dtype, int* kernel, int* nprtn)
+int blco_select_kernel(blco_dev* d, int mode, int rank, int factor_dtype, int* kernel, int* nprtn, int* thread_cf)
 {
-    if (!d || !kernel || !nprtn) return fail(BLCO_EINVAL, "null argument");
+    if (!d || !kernel || !nprtn || !thread_cf) return fail(BLCO_EINVAL, "null argument");
     if (mode < 0 || mode >= d->nmodes) return fail(BLCO_EINVAL, "mode %d out of range", mode);
     if (rank < 1) return fail(BLCO_EINVAL, "rank %d", rank);
-    if (!select_kernel(d, mode, rank, factor_dtype, kernel, nprtn)) {
+    if (!select_kernel(d, mode, rank, factor_dtype, kernel, nprtn, thread_cf)) {
         *kernel = auto_kernel(d, mode, rank, factor_dtype);
         *nprtn = 0;
+        *thread_cf = 0;
     }
     return BLCO_OK;
 }

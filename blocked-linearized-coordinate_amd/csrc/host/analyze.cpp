@@ -14,6 +14,12 @@ namespace blco {
 
 constexpr uint64_t LOCAL_ROWS = 1u << 16;
 
+// The block holding nonzero i (blocks are contiguous, in order).
+uint64_t block_of(const uint64_t* block_start, uint64_t nblocks, uint64_t i)
+{
+    return (uint64_t)(std::upper_bound(block_start, block_start + nblocks, i) - block_start) - 1;
+}
+
 int analyze_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks, const int* pos, uint64_t nnz,
                    const uint64_t* coords, uint64_t nblocks, const uint64_t* block_start,
                    const uint64_t* block_nnz, const uint64_t* block_coords, uint32_t fine_chunk, uint32_t hot_max,
@@ -42,38 +48,50 @@ int analyze_tensor(int nmodes, const uint64_t* dims, const uint64_t* masks, cons
         // the lowest LOCAL_ROWS rows (where skewed modes keep their hot rows)
         // are counted per thread and merged, the rest with atomics
         const uint64_t local = std::min<uint64_t>(dm, LOCAL_ROWS);
-        for (uint64_t b = 0; b < nblocks; ++b) {
-            const uint64_t bc = block_coords[b * nmodes + m];
-            const uint64_t lo = block_start[b], hi = lo + block_nnz[b];
-            uint64_t worst = 0;
-            bool out = false;
-            #pragma omp parallel reduction(max : worst) reduction(|| : out) if (par)
+        uint64_t worst = 0;
+        bool out = false;
+        if (par) {
+            // each thread a contiguous range of nonzeros, walking the blocks
+            // it crosses (one parallel region per mode, whatever the block
+            // count)
+            #pragma omp parallel reduction(max : worst) reduction(|| : out)
             {
-                std::vector<uint32_t> mine(par && count ? local : 0, 0);
-                #pragma omp for schedule(static)
-                for (uint64_t i = lo; i < hi; ++i) {
-                    const uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                std::vector<uint32_t> mine(count ? local : 0, 0);
+                const uint64_t nt = (uint64_t)omp_get_num_threads(), th = (uint64_t)omp_get_thread_num();
+                const uint64_t i0 = nnz * th / nt, i1 = nnz * (th + 1) / nt;
+                uint64_t b = i0 < i1 ? block_of(block_start, nblocks, i0) : 0;
+                for (uint64_t i = i0; i < i1; ++i) {
+                    while (i >= block_start[b] + block_nnz[b]) ++b;
+                    const uint64_t v = ((coords[i] >> ps) & mk) | block_coords[b * nmodes + m];
                     if (v >= dm) {
                         out = true;
                         worst = v > worst ? v : worst;
                         continue;
                     }
                     if (!count) continue;
-                    if (!par) {
-                        if (c[v] != 0xFFFFFFFFu) ++c[v];
-                    } else if (v < local) {
-                        ++mine[v];
-                    } else {
-                        __atomic_fetch_add(&c[v], 1u, __ATOMIC_RELAXED);
-                    }
+                    if (v < local) ++mine[v];
+                    else __atomic_fetch_add(&c[v], 1u, __ATOMIC_RELAXED);
                 }
                 for (uint64_t v = 0; v < mine.size(); ++v)
                     if (mine[v]) __atomic_fetch_add(&c[v], mine[v], __ATOMIC_RELAXED);
             }
-            if (out)
-                return fail(BLCO_EINVAL, "block %llu decodes mode-%d index %llu >= %llu", (unsigned long long)b, m,
-                            (unsigned long long)worst, (unsigned long long)dm);
+        } else {
+            for (uint64_t b = 0; b < nblocks; ++b) {
+                const uint64_t bc = block_coords[b * nmodes + m];
+                for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
+                    const uint64_t v = ((coords[i] >> ps) & mk) | bc;
+                    if (v >= dm) {
+                        out = true;
+                        worst = v > worst ? v : worst;
+                    } else if (count && c[v] != 0xFFFFFFFFu) {
+                        ++c[v];
+                    }
+                }
+            }
         }
+        if (out)
+            return fail(BLCO_EINVAL, "a block decodes mode-%d index %llu >= %llu", m, (unsigned long long)worst,
+                        (unsigned long long)dm);
         if (count && nnz) {
             const uint64_t thr = std::max<uint64_t>(2, nnz / 2048);
             std::vector<uint32_t> cand;

@@ -59,15 +59,18 @@ std::vector<uint64_t> sample_keys(uint64_t nblocks, const uint64_t* block_start,
                                   uint64_t S, K&& key)
 {
     std::vector<std::vector<uint64_t>> parts(omp_get_max_threads());
+    const uint64_t nnz = nblocks ? block_start[nblocks - 1] + block_nnz[nblocks - 1] : 0;
     #pragma omp parallel
     {
         std::vector<uint64_t>& mine = parts[omp_get_thread_num()];
-        for (uint64_t b = 0; b < nblocks; ++b) {
-            #pragma omp for schedule(static) nowait
-            for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
-                const uint64_t k = key(b, i);
-                if (S == 1 || mix64(k) % S == 0) mine.push_back(k);
-            }
+        const uint64_t nt = (uint64_t)omp_get_num_threads(), th = (uint64_t)omp_get_thread_num();
+        const uint64_t i0 = (uint64_t)((unsigned __int128)nnz * th / nt);
+        const uint64_t i1 = (uint64_t)((unsigned __int128)nnz * (th + 1) / nt);
+        uint64_t b = i0 < i1 ? block_of(block_start, nblocks, i0) : 0;
+        for (uint64_t i = i0; i < i1; ++i) {
+            while (i >= block_start[b] + block_nnz[b]) ++b;
+            const uint64_t k = key(b, i);
+            if (S == 1 || mix64(k) % S == 0) mine.push_back(k);
         }
     }
     std::vector<uint64_t> keys;
@@ -99,19 +102,21 @@ void compute_static_features(int N, const uint64_t* dims, const uint64_t* masks,
                 // lowest rows (where skewed modes keep their hot rows) counted
                 // per thread and merged, the rest with atomic increments
                 const uint64_t local = std::min<uint64_t>(dims[n], 1u << 16);
-                for (uint64_t b = 0; b < nblocks; ++b) {
-                    #pragma omp parallel
-                    {
-                        std::vector<uint32_t> mine(local, 0);
-                        #pragma omp for schedule(static)
-                        for (uint64_t i = block_start[b]; i < block_start[b] + block_nnz[b]; ++i) {
-                            const uint64_t v = index(b, i, n);
-                            if (v < local) ++mine[v];
-                            else __atomic_fetch_add(&cp[v], 1u, __ATOMIC_RELAXED);
-                        }
-                        for (uint64_t v = 0; v < local; ++v)
-                            if (mine[v]) __atomic_fetch_add(&cp[v], mine[v], __ATOMIC_RELAXED);
+                #pragma omp parallel
+                {
+                    // a contiguous range of nonzeros per thread, walking its blocks
+                    std::vector<uint32_t> mine(local, 0);
+                    const uint64_t nt = (uint64_t)omp_get_num_threads(), th = (uint64_t)omp_get_thread_num();
+                    const uint64_t i0 = nnz * th / nt, i1 = nnz * (th + 1) / nt;
+                    uint64_t b = i0 < i1 ? block_of(block_start, nblocks, i0) : 0;
+                    for (uint64_t i = i0; i < i1; ++i) {
+                        while (i >= block_start[b] + block_nnz[b]) ++b;
+                        const uint64_t v = index(b, i, n);
+                        if (v < local) ++mine[v];
+                        else __atomic_fetch_add(&cp[v], 1u, __ATOMIC_RELAXED);
                     }
+                    for (uint64_t v = 0; v < local; ++v)
+                        if (mine[v]) __atomic_fetch_add(&cp[v], mine[v], __ATOMIC_RELAXED);
                 }
             } else {
                 for (uint64_t b = 0; b < nblocks; ++b)

@@ -13,6 +13,8 @@ namespace blco {
 // A line on stderr when BLCO_PROGRESS is set (long host phases of large
 // tensors: generation, encoding, row-ordered copies).
 void progress(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+// The block holding nonzero i (blocks are contiguous, in order; i < nnz).
+uint64_t block_of(const uint64_t* block_start, uint64_t nblocks, uint64_t i);
 
 // Sets the thread's last-error message and returns `code`.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));

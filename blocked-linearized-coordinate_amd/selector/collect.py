@@ -1,15 +1,20 @@
 """Timing data for the learned MTTKRP kernel selector.
 
-Tensor t of the selector set (t = 0..399 training, 400..499 held out) is
-drawn from seed 1004 * 100000 + t: order uniform in {3, 4, 5}; mode lengths
-log-uniform in [1e2, 1e7]; nonzeros log-uniform in [1e5, nnz_max]
-(default 1e7; duplicates merged, no top-up); per-mode Zipf exponent uniform
-in [0, 1.5]; rank from {8, 16, 32, 64}; FP32 values (Poisson counts of a
-rank-8 model).  For every mode, every candidate that can run is timed with
-HIP events (blco_mttkrp_time) on factors of ones, and the features of the
-mode are recorded.
+Tensor t of the selector set is drawn from seed 1004 * 100000 + t: order
+uniform in {3, 4, 5}; mode lengths log-uniform in [1e2, 1e7]; nonzeros
+log-uniform in [nnz_min, nnz_max] (duplicates merged, no top-up); per-mode
+Zipf exponent uniform in [0, 1.5]; rank from {8, 16, 32, 64}; FP32 values
+(Poisson counts of a rank-8 model).  Tensors 0..499 take nonzeros in [1e5,
+1e7] (0..399 training, 400..499 held out), tensors 500..539 in [1e7, 1e8]
+(500..529 training, 530..539 held out).  Each tensor is generated, encoded
+and made resident on the GPU (blco_generate_device, the same tensor as the
+host generator's; keys wider than it takes are generated on the host).  For every mode, every candidate that can run (kernel,
+lvl3 partitions, tiles per wave) is timed with HIP events
+(blco_mttkrp_time) on factors of ones, and the features of the mode are
+recorded.
 
   python collect.py --first 0 --count 100 --out data/part0.npz
+  python collect.py --first 500 --count 40 --nnz-min 1e7 --nnz-max 1e8 --out data/large.npz
 
 Writes the file after every tensor, so an interrupted run keeps what it
 measured.  Prints one line per tensor.
@@ -39,11 +44,11 @@ def load_pkg():
 BASE_SEED = 1004 * 100000
 
 
-def tensor_spec(t, nnz_max):
+def tensor_spec(t, nnz_min, nnz_max):
     rng = np.random.default_rng(BASE_SEED + t)
     order = int(rng.integers(3, 6))
     dims = [int(round(10 ** rng.uniform(2, 7))) for _ in range(order)]
-    nnz = int(round(10 ** rng.uniform(5, np.log10(nnz_max))))
+    nnz = int(round(10 ** rng.uniform(np.log10(nnz_min), np.log10(nnz_max))))
     zipf = [float(rng.uniform(0, 1.5)) for _ in range(order)]
     rank = int(rng.choice([8, 16, 32, 64]))
     return dict(t=t, seed=BASE_SEED + t, dims=dims, nnz=nnz, zipf=zipf, rank=rank)
@@ -60,19 +65,19 @@ def ones(B, rows, rank):
 def rule_candidate(B, dev, mode, rank, nprtn_rule):
     k = dev.auto_kernel(mode, rank)
     for c in range(B.SEL_NCAND):
-        kc, nc = B.sel_candidate(c)
-        if kc == k and (k != B.K_LVL3 or nc == nprtn_rule):
+        kc, nc, tc = B.sel_candidate(c)
+        if kc == k and tc == 0 and (k != B.K_LVL3 or nc == nprtn_rule):
             return c
     raise RuntimeError("rule kernel %d is not a candidate" % k)
 
 
 def time_candidate(B, dev, mode, rank, F, out, c):
-    k, np_ = B.sel_candidate(c)
-    ms, used = B.mttkrp_time(dev, mode, rank, F, out, B.F32, k, np_, reps=1)
+    k, np_, tpw = B.sel_candidate(c)
+    ms, used = B.mttkrp_time(dev, mode, rank, F, out, B.F32, k, np_, reps=1, thread_cf=tpw)
     if used != k:
         raise RuntimeError("candidate %d ran kernel %d" % (c, used))
     if ms < 20.0:  # short: average more launches
-        ms, _ = B.mttkrp_time(dev, mode, rank, F, out, B.F32, k, np_, reps=5)
+        ms, _ = B.mttkrp_time(dev, mode, rank, F, out, B.F32, k, np_, reps=5, thread_cf=tpw)
     return ms
 
 
@@ -80,11 +85,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--first", type=int, default=0)
     ap.add_argument("--count", type=int, default=500)
+    ap.add_argument("--nnz-min", type=float, default=1e5)
     ap.add_argument("--nnz-max", type=float, default=1e7)
     ap.add_argument("--nprtn-rule", type=int, default=16,
                     help="lvl3 partitions of the -k 10 rule (the reference's default is the "
                          "OpenMP thread count)")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--verbose", action="store_true", help="one line per timed candidate (stderr)")
     args = ap.parse_args()
     B = load_pkg()
     if B.device_count() < 1:
@@ -93,21 +100,31 @@ def main():
     meta = []
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     for t in range(args.first, args.first + args.count):
-        sp = tensor_spec(t, args.nnz_max)
+        sp = tensor_spec(t, args.nnz_min, args.nnz_max)
         t0 = time.perf_counter()
-        coo = B.Coo.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
-                             seed=sp["seed"], topup=False)
-        h = B.encode(coo)
-        dev = B.DeviceTensor(h, value_dtype=1)
+        try:
+            dev = B.DeviceTensor.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                                          seed=sp["seed"], topup=False, value_dtype=1)
+        except B.BlcoError:  # keys wider than the GPU generator takes: the host generator
+            coo = B.Coo.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                                 seed=sp["seed"], topup=False)
+            dev = B.DeviceTensor(B.encode(coo), value_dtype=1)
+            del coo
         R = sp["rank"]
         F = [ones(B, d, R) for d in sp["dims"]]
         tgen = time.perf_counter() - t0
+        if args.verbose:
+            print("  tensor %d: set up in %.1f s" % (t, tgen), file=sys.stderr, flush=True)
         for m in range(len(sp["dims"])):
             out = B.DeviceArray((sp["dims"][m], R), B.F32)
             tm = np.full(B.SEL_NCAND, np.nan)
             for c in range(B.SEL_NCAND):
                 if dev.candidate_valid(m, R, B.F32, c):
+                    tc = time.perf_counter()
                     tm[c] = time_candidate(B, dev, m, R, F, out, c)
+                    if args.verbose:
+                        print("  tensor %d mode %d candidate %d: %.3f ms (%.1f s)"
+                              % (t, m, c, tm[c], time.perf_counter() - tc), file=sys.stderr, flush=True)
             rows["feat"].append(dev.features(m, R, B.F32))
             rows["times"].append(tm)
             rows["tensor"].append(t)
@@ -115,7 +132,7 @@ def main():
             rows["rank"].append(R)
             rows["rule"].append(rule_candidate(B, dev, m, R, args.nprtn_rule))
             out.free()
-        sp.update(nnz_unique=int(coo.nnz), blocks=int(h.nblocks))
+        sp.update(nnz_unique=int(dev.nnz), blocks=int(dev.nblocks))
         meta.append(sp)
         for f in F:
             f.free()
@@ -126,7 +143,7 @@ def main():
                  meta=np.array(json.dumps(meta)))
         best = [np.nanmin(x) for x in rows["times"][-len(sp["dims"]):]]
         print("tensor %d: order %d nnz %d blocks %d rank %d setup %.1f s total %.1f s best ms %s"
-              % (t, len(sp["dims"]), coo.nnz, h.nblocks, R, tgen, time.perf_counter() - t0,
+              % (t, len(sp["dims"]), dev.nnz, dev.nblocks, R, tgen, time.perf_counter() - t0,
                  np.round(best, 3).tolist()), flush=True)
 
 

@@ -1,12 +1,12 @@
 """Trains the learned MTTKRP kernel selector on timings from collect.py and
 reports its regret.
 
-The network (24 features -> 64 -> 64 -> 11 predicted log2 times) is trained
+The network (24 features -> 64 -> 64 -> 15 predicted log2 times) is trained
 on the GPU by the library's own kernels (blco_mlp_train: forward, masked MSE,
-backward, Adam) on the training tensors (t < 400); the candidates that cannot
-run for a mode are masked out of the loss.  On the held-out tensors
-(t >= 400) it reports, against the exhaustive choice (the fastest measured
-candidate of each mode):
+backward, Adam) on the training tensors (is_train); the candidates that cannot
+run for a mode are masked out of the loss.  On the held-out tensors it
+reports, against the exhaustive choice (the fastest measured candidate of
+each mode):
 
   regret_total = sum(time of the choice) / sum(fastest time) - 1
   regret_mean  = mean over modes of (time of the choice / fastest - 1)
@@ -62,6 +62,13 @@ def regret(times, choice):
                 time_ms=float(got.sum()), best_ms=float(best.sum()))
 
 
+def is_train(t):
+    """Training tensors: 0..399 of the 1e5-1e7-nonzero set and 500..529 of
+    the 1e7-1e8-nonzero set; the rest are held out."""
+    t = np.asarray(t)
+    return (t < N_TRAIN) | ((t >= 500) & (t < 530))
+
+
 def choose(pred, valid):
     p = np.where(valid, pred, np.inf)
     return np.argmin(p, axis=1)
@@ -82,7 +89,7 @@ def main():
     X = D["feat"].astype(np.float64)
     valid = ~np.isnan(D["times"])
     logt = np.where(valid, np.log2(np.where(valid, D["times"], 1.0)), 0.0)
-    tr = D["tensor"] < N_TRAIN
+    tr = is_train(D["tensor"])
     te = ~tr
     mu = X[tr].mean(axis=0)
     sd = X[tr].std(axis=0)
@@ -100,7 +107,8 @@ def main():
                n_train_tensors=int(len(set(D["tensor"][tr].tolist()))),
                n_test_tensors=int(len(set(D["tensor"][te].tolist()))),
                loss_first=float(trace[0]), loss_last=float(trace[-1]))
-    for name, sel in (("train", tr), ("heldout", te)):
+    big = D["tensor"] >= 500
+    for name, sel in (("train", tr), ("heldout", te), ("heldout_1e5_1e7", te & ~big), ("heldout_1e7_1e8", te & big)):
         if not sel.any():
             continue
         T = D["times"][sel]

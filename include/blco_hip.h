@@ -340,7 +340,7 @@ int blco_batch_cpals(blco_batch* b, const blco_cpals_opts* opts, double* const* 
  * training data. */
 int blco_mttkrp_time(blco_dev* d, int mode, int rank, int factor_dtype,
                      void* const* factors, void* out, int kernel, int nprtn,
-                     int reps, float* ms, int* used_kernel);
+                     int thread_cf, int reps, float* ms, int* used_kernel);
 
 /* ---- a small MLP run and trained with the library's own HIP kernels ----
  * (no reference counterpart: the reference's kernel choice is the fixed
@@ -376,7 +376,10 @@ int blco_mlp_train(blco_mlp* h, const float* X, const float* T, const float* M,
 /* ---- learned MTTKRP kernel selector (kernel id 20) ----
  * Candidates c = 0..BLCO_SEL_NCAND-1: 0 row window (12), 1 segmented lvl1
  * (11), 2 atomic (13), 3..9 lvl3 with 1, 2, 4, 8, 16, 32, 64 partitions,
- * 10 row-ordered (14).
+ * 10 row-ordered (14), and the thread coarsening of the reference's
+ * --thread-cf (tiles of 64 nonzeros per wave, src/main.cpp:127) as
+ * candidates of their own: 11, 12 segmented lvl1 with 1 and 16 tiles per
+ * wave (4 otherwise), 13, 14 row-ordered with 8 and 32 (16 otherwise).
  * The network maps BLCO_SEL_NFEAT features (normalised by the installed mean
  * and scale) to predicted log2 times of every candidate; the smallest
  * prediction among the candidates that can run is taken.  With no network
@@ -384,8 +387,10 @@ int blco_mlp_train(blco_mlp* h, const float* X, const float* T, const float* M,
  * (lib/../selector/selector.bin, or $BLCO_SELECTOR) is installed on first
  * use. */
 #define BLCO_SEL_NFEAT 24
-#define BLCO_SEL_NCAND 11
-int blco_sel_candidate(int c, int* kernel, int* nprtn);
+#define BLCO_SEL_NCAND 15
+/* Candidate c -> kernel id, lvl3 partitions, tiles per wave (0: the
+ * kernel's default). */
+int blco_sel_candidate(int c, int* kernel, int* nprtn, int* thread_cf);
 /* 1 when candidate c can run for (mode, rank, factor type), else 0. */
 int blco_sel_valid(const blco_dev* d, int mode, int rank, int factor_dtype, int c);
 int blco_mode_features(const blco_dev* d, int mode, int rank, int factor_dtype,
@@ -395,10 +400,10 @@ int blco_selector_install(int nlayers, const int* widths, const float* params,
                           const float* xmean, const float* xstd);
 int blco_selector_load(const char* path);
 int blco_selector_installed(void);
-/* The kernel id 20 would run for (mode, rank, factor type) and its lvl3
- * partition count (0 = the caller's). */
+/* The kernel id 20 would run for (mode, rank, factor type), its lvl3
+ * partition count and tiles per wave (0 = the caller's). */
 int blco_select_kernel(blco_dev* d, int mode, int rank, int factor_dtype,
-                       int* kernel, int* nprtn);
+                       int* kernel, int* nprtn, int* thread_cf);
 
 #ifdef __cplusplus
 }

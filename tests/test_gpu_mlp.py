@@ -115,12 +115,14 @@ def test_selector_install_and_choice(B, gpu):
 
     try:
         install([6, 2])  # lvl3 with 8 partitions, then atomic
-        assert dev.select_kernel(0, 16, B.F32) == (B.K_LVL3, 8)
-        assert dev.select_kernel(1, 16, B.F32) == (B.K_ATOMIC, 1)  # lvl3 cannot hold 3000 rows
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_LVL3, 8, 0)
+        assert dev.select_kernel(1, 16, B.F32) == (B.K_ATOMIC, 1, 0)  # lvl3 cannot hold 3000 rows
         install([1])
-        assert dev.select_kernel(0, 16, B.F32) == (B.K_BATCHED, 1)
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_BATCHED, 1, 0)
+        install([12])  # segmented lvl1 with 16 tiles per wave
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_BATCHED, 1, 16)
         B.selector_install(None, None, None, None)
-        assert dev.select_kernel(1, 16, B.F32) == (dev.auto_kernel(1, 16), 0)
+        assert dev.select_kernel(1, 16, B.F32) == (dev.auto_kernel(1, 16), 0, 0)
     finally:
         B.selector_install(None, None, None, None)
     dev.free()
@@ -135,17 +137,18 @@ def test_timed_mttkrp_reports_the_kernel(B, gpu):
     for c in range(B.SEL_NCAND):
         if not dev.candidate_valid(0, 16, B.F32, c):
             continue
-        k, np_ = B.sel_candidate(c)
-        ms, used = B.mttkrp_time(dev, 0, 16, F, out, B.F32, k, np_, reps=2)
+        k, np_, tpw = B.sel_candidate(c)
+        ms, used = B.mttkrp_time(dev, 0, 16, F, out, B.F32, k, np_, reps=2, thread_cf=tpw)
         assert ms > 0 and used == k
     dev.free()
 
 
 def test_selector_training_from_committed_timings(B, gpu):
     """Trains the selector network with the library's kernels on the
-    committed timings of the 400 training tensors and checks the choice on
-    the 100 held-out ones: total-time regret against the exhaustive choice
-    <= 10%, and below the -k 10 rule's."""
+    committed timings of the training tensors (0..399 and 500..529) and
+    checks the choice on the held-out ones (400..499 and 530..539):
+    total-time regret against the exhaustive choice <= 10%, and below the
+    -k 10 rule's."""
     import glob
     sel = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "blocked-linearized-coordinate_amd", "selector", "data")
@@ -153,7 +156,9 @@ def test_selector_training_from_committed_timings(B, gpu):
     D = {k: np.concatenate([p[k] for p in parts]) for k in ("feat", "times", "tensor", "rule")}
     valid = ~np.isnan(D["times"])
     logt = np.where(valid, np.log2(np.where(valid, D["times"], 1.0)), 0.0).astype(np.float32)
-    tr, te = D["tensor"] < 400, D["tensor"] >= 400
+    t = D["tensor"]
+    tr = (t < 400) | ((t >= 500) & (t < 530))
+    te = ~tr
     X = D["feat"].astype(np.float64)
     mu, sd = X[tr].mean(axis=0), X[tr].std(axis=0)
     sd = np.where(sd > 1e-6, sd, 1.0)
