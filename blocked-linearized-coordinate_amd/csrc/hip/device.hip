@@ -1057,6 +1057,8 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
             a.chunk0 = 0;
             a.nchunks = stab->n;
             const size_t lds = rows_kernel_plan(d, mode, rank, ftype, kind == W_FIBER, &a);
+            // the form with LDS windows only where the plan holds some
+            if (lds) kind = kind == W_FIBER ? W_FIBER_LDS : W_SORTED_LDS;
             const uint32_t per_wg = WAVES_PER_WG * a.cpw;
             if (stab->n) rc = launch_mttkrp(kind, a, ftype, (stab->n + per_wg - 1) / per_wg, lds, st);
         } else {

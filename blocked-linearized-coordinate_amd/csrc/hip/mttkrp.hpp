@@ -76,7 +76,7 @@ struct RowsPlan {
 };
 
 enum { W_LVL1 = 0, W_LVL3 = 1, W_ATOMIC = 2, W_WINDOW = 3, W_WINDOW64 = 4, W_SORTED = 5, W_SORTED64 = 6,
-       W_FIBER = 7, W_FIBER64 = 8 };
+       W_FIBER = 7, W_FIBER64 = 8, W_SORTED_LDS = 9, W_FIBER_LDS = 10 };
 
 // The row-ordered and fiber kernels take 31-bit byte offsets into factors
 // of fewer bytes than this (bit 31 marks an offset into an LDS window).

@@ -830,7 +830,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_fiber(MttkrpArgs a)
 //   starts a fiber, and their product scales the fiber's terms.
 // The flush and the final segmented scan over the slots are those of
 // k_mttkrp_sorted.
-template <int N, typename FT, typename VT, int C, int L, bool EXACT, bool FIB>
+template <int N, typename FT, typename VT, int C, int L, bool EXACT, bool FIB, bool HOT>
 __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
 {
     using AT = typename Acc<FT>::type;
@@ -860,9 +860,9 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
         for (int m = 0; m < N; ++m)
             if (m != mode && !(FIB && m == a.inner) && j < N - 1) in[j++] = m;
     }
-    constexpr int NT = N - 1 < 2 ? N - 1 : 2;  // input modes with a window
-    uint32_t tab_off[NT];
-    {
+    constexpr int NT = !HOT ? 0 : N - 1 < 2 ? N - 1 : 2;  // input modes with a window
+    uint32_t tab_off[NT > 0 ? NT : 1] = {};
+    if (HOT) {
         uint32_t o = 0;
         #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
             o += nb;
         }
     }
-    __syncthreads();  // the only block barrier: waves leave independently below
+    if (HOT) __syncthreads();  // the only block barrier: waves leave independently below
     const int slot = lane / L, sub = lane % L;
     AT* out = reinterpret_cast<AT*>(a.out);
     const VT* vals = reinterpret_cast<const VT*>(a.vals);
@@ -917,7 +917,7 @@ __global__ __launch_bounds__(WG) void k_mttkrp_rows(MttkrpArgs a)
         // a row piece from the factor, or (offset with bit 31 set) from its
         // LDS window
         auto row_piece = [&](int m, uint32_t byte_off, AT (&x)[C]) {
-            if (byte_off >> 31) {
+            if (HOT && (byte_off >> 31)) {
                 const unsigned char* t = s_tab + (byte_off & 0x7fffffffu);
                 if (EXACT || col + C <= R) {
                     load_row<FT, C>(reinterpret_cast<const FT*>(t + colb), x);
@@ -1179,12 +1179,14 @@ inline const void* kernel_ptr(int which)
     if (which == W_LVL3) return (const void*)&k_mttkrp_lvl3<N, FT, VT, C, L, EX>;
     if (which == W_WINDOW) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint32_t>;
     if (which == W_WINDOW64) return (const void*)&k_mttkrp_window<N, FT, VT, C, L, EX, uint64_t>;
-    if (which == W_SORTED) return (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, false>;
+    if (which == W_SORTED) return (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, false, false>;
+    if (which == W_SORTED_LDS) return (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, false, true>;
     if (which == W_SORTED64) return (const void*)&k_mttkrp_sorted<N, FT, VT, C, L, EX, uint64_t>;
-    if (which == W_FIBER || which == W_FIBER64) {
+    if (which == W_FIBER || which == W_FIBER64 || which == W_FIBER_LDS) {
         if constexpr (N >= 3)
-            return which == W_FIBER ? (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, true>
-                                    : (const void*)&k_mttkrp_fiber<N, FT, VT, C, L, EX, uint64_t>;
+            return which == W_FIBER       ? (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, true, false>
+                   : which == W_FIBER_LDS ? (const void*)&k_mttkrp_rows<N, FT, VT, C, L, EX, true, true>
+                                          : (const void*)&k_mttkrp_fiber<N, FT, VT, C, L, EX, uint64_t>;
         return nullptr;  // an order-2 tensor has no fibers
     }
     return (const void*)&k_mttkrp_atomic<N, FT, VT, C, L, EX>;

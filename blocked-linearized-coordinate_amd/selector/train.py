@@ -11,7 +11,8 @@ each mode):
   regret_total = sum(time of the choice) / sum(fastest time) - 1
   regret_mean  = mean over modes of (time of the choice / fastest - 1)
 
-for the learned choice and for the -k 10 rule, and writes the selector file
+for the library's learned choice (choose_lib), the network's plain argmin
+and the -k 10 rule, and writes the selector file
 that the library loads (selector.bin next to this script by default).
 
   python train.py data/part0.npz data/part1.npz ... --out selector.bin --report report.json
@@ -74,6 +75,21 @@ def choose(pred, valid):
     return np.argmin(p, axis=1)
 
 
+# The library's decision (csrc/hip/selector.hip): the rule's candidate unless
+# the network predicts 2^MARGIN times less time, and the rule outright for
+# modes whose size features lie beyond SIZE_Z training standard deviations.
+MARGIN, SIZE_Z, SIZE_FEATS = 0.1, 4.0, [0, 1, 2, 3, 4, 7, 16, 22]
+
+
+def choose_lib(pred, valid, rule, Xn):
+    best = choose(pred, valid)
+    idx = np.arange(len(pred))
+    inside = (np.abs(Xn[:, SIZE_FEATS]) <= SIZE_Z).all(axis=1)
+    rule_ok = valid[idx, rule]
+    gain = np.where(rule_ok, pred[idx, rule] - pred[idx, best], np.inf)
+    return np.where(inside & (gain > MARGIN), best, np.where(rule_ok, rule, best))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("data", nargs="+")
@@ -112,7 +128,8 @@ def main():
         if not sel.any():
             continue
         T = D["times"][sel]
-        rep[name] = dict(learned=regret(T, choose(pred[sel], valid[sel])),
+        rep[name] = dict(learned=regret(T, choose_lib(pred[sel], valid[sel], D["rule"][sel], Xn[sel])),
+                         learned_argmin=regret(T, choose(pred[sel], valid[sel])),
                          rule_k10=regret(T, D["rule"][sel]),
                          always_window=regret(T, np.zeros(sel.sum(), int)),
                          always_lvl1=regret(T, np.ones(sel.sum(), int)),
