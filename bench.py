@@ -18,7 +18,9 @@ MTTKRP time of each mode is event-timed on the same stream.
 
 Other workloads: nell2-shape-77M (12092 x 9184 x 28818, 77M nonzeros, with
 the per-mode MTTKRP error of the FP32 form against the FP64 oracle),
-small-1M, batched-64 (64 small tensors in one persistent kernel).
+small-1M, batched-64 (64 small tensors in one persistent kernel),
+selector-heldout (MTTKRP of the selector's 110 held-out tensors under the
+learned selector against the -k 10 rule).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -441,6 +443,87 @@ def run_batched(B, args):
     print(json.dumps(out), flush=True)
 
 
+def heldout_specs():
+    """The held-out tensors of the selector's timing set (the training never
+    saw them): 400..499 (1e5-1e7 nonzeros) and 530..539 (1e7-1e8), drawn as
+    the selector's collection draws them."""
+    import importlib.util
+    path = os.path.join(ROOT, "blocked-linearized-coordinate_amd", "selector", "collect.py")
+    spec = importlib.util.spec_from_file_location("blco_selector_collect", path)
+    col = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(col)
+    return ([col.tensor_spec(t, 1e5, 1e7) for t in range(400, 500)]
+            + [col.tensor_spec(t, 1e7, 1e8) for t in range(530, 540)])
+
+
+def run_selector_heldout(B, args):
+    """MTTKRP time of every mode of the held-out selector tensors (FP32 values
+    and factors) under the learned selector (-k 20) against the -k 10 rule
+    (lvl3 with 16 partitions, the rule's set-up in the selector's timing
+    data), each event-timed over --steps launches after one untimed call."""
+    if not B.selector_installed():
+        sys.exit("selector-heldout needs the selector file")
+    reps = max(args.steps, 1)
+    t_start = time.perf_counter()
+    tot = {"all": [0.0, 0.0], "1e5-1e7": [0.0, 0.0], "1e7-1e8": [0.0, 0.0]}
+    nmodes = changed = 0
+    worst = []
+    for sp in heldout_specs():
+        try:
+            dev = B.DeviceTensor.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                                          seed=sp["seed"], topup=False, value_dtype=1)
+        except B.BlcoError:  # keys wider than the GPU generator takes
+            coo = B.Coo.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                                 seed=sp["seed"], topup=False)
+            dev = B.DeviceTensor(B.encode(coo), value_dtype=1)
+            del coo
+        R = sp["rank"]
+        F = []
+        for d in sp["dims"]:
+            a = B.DeviceArray((d, R), B.F32)
+            h = np.ones((d, R), np.float32)
+            B._check(B.lib().blco_memcpy_htod(a.ptr, h.ctypes.data, a.nbytes))
+            F.append(a)
+        rng = "1e7-1e8" if sp["t"] >= 500 else "1e5-1e7"
+        line = []
+        for m in range(len(sp["dims"])):
+            out = B.DeviceArray((sp["dims"][m], R), B.F32)
+            ms_r, _ = B.mttkrp_time(dev, m, R, F, out, B.F32, B.K_AUTO, 16, reps=reps)
+            ms_s, _ = B.mttkrp_time(dev, m, R, F, out, B.F32, B.K_SELECT, 16, reps=reps)
+            ch = list(dev.select_kernel(m, R, B.F32))
+            if ch[0] != dev.auto_kernel(m, R) or ch[2] != 0 or (ch[0] == B.K_LVL3 and ch[1] != 16):
+                changed += 1
+            for k in ("all", rng):
+                tot[k][0] += ms_r
+                tot[k][1] += ms_s
+            worst.append((ms_s / ms_r, sp["t"], m, ch))
+            line.append("%.3f/%.3f" % (ms_r, ms_s))
+            nmodes += 1
+            out.free()
+        for a in F:
+            a.free()
+        log("tensor %d (%s, %d nonzeros, rank %d): rule/selector ms %s"
+            % (sp["t"], rng, dev.nnz, R, " ".join(line)))
+        dev.free()
+    worst.sort(reverse=True)
+    res = {k: {"rule_ms": round(v[0], 3), "selector_ms": round(v[1], 3),
+               "gain": round(v[0] / v[1] - 1.0, 4) if v[1] > 0 else None} for k, v in tot.items()}
+    out = {
+        "metric": "MTTKRP ms summed over every mode of the selector's held-out tensors, 1 GPU",
+        "value": res["all"]["selector_ms"], "unit": "ms", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": res["all"]["selector_ms"], "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32 values and factors",
+        "data": "synthetic (the selector's held-out tensors: seeds 100400400..100400499, 100400530..100400539)",
+        "config": {"workload": "selector-heldout", "tensors": 110, "modes": nmodes,
+                   "parallelism": "single GPU"},
+        "rule_ms": res["all"]["rule_ms"], "gain_vs_rule": res["all"]["gain"], "per_range": res,
+        "modes_changed_by_selector": changed,
+        "worst_modes": [dict(tensor=t, mode=m, choice=c, ratio=round(r, 3)) for r, t, m, c in worst[:5]],
+        "host_wall_s": round(time.perf_counter() - t_start, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_workload(B, args):
     """nell2-shape-77M or small-1M: host-generated, both forms, the per-mode
     MTTKRP error of the FP32 form against the FP64 oracle, CPU baseline on
@@ -529,13 +612,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large-1.7B",
-                    choices=["large-1.7B", "batched-64"] + sorted(WORKLOADS))
+                    choices=["large-1.7B", "batched-64", "selector-heldout"] + sorted(WORKLOADS))
     ap.add_argument("--large-scale", type=float, default=1.0,
                     help="large-1.7B: fraction of the 1.7e9 nonzeros to generate")
     ap.add_argument("--large-forms", default="",
                     help="large-1.7B: comma list of fp64,fp32,bf16,fp32_streamed (default fp64,fp32)")
-    ap.add_argument("--slot-nnz", type=int, default=1 << 26,
-                    help="large-1.7B streamed form: nonzeros per device slot (two slots)")
+    ap.add_argument("--slot-nnz", type=int, default=0,
+                    help="large-1.7B streamed form: nonzeros per device slot (two slots; 0: the "
+                         "library's default, 2^24)")
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="timed CPU iterations (0: 1 on large-1.7B, 2 otherwise)")
     ap.add_argument("--cpu-sample-nnz", type=int, default=20_000_000,
@@ -562,6 +646,8 @@ def main():
         args.cpu_iters = 1 if args.workload == "large-1.7B" else 2
     if args.workload == "batched-64":
         run_batched(B, args)
+    elif args.workload == "selector-heldout":
+        run_selector_heldout(B, args)
     elif args.workload == "large-1.7B":
         if args.no_fp64:
             args.large_forms = args.large_forms or "fp32"

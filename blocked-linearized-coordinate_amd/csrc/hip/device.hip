@@ -32,12 +32,31 @@ namespace blco {
 constexpr uint64_t LDS_OUT_MAX = 48 * 1024;
 constexpr int LVL3_MAX_MODE_LENGTH = 100;     // src/utils.hpp:13
 constexpr int DEFAULT_TPW = 4;                // tiles of 64 per wave
-constexpr int SORTED_TPW = 16;                // tiles of 64 per wave, row-ordered kernel
+constexpr int SORTED_TPW = 16;                // tiles of 64 per wave, row-ordered kernel (most)
+constexpr uint64_t SORTED_TARGET_CHUNKS = 8192;
 constexpr int DEFAULT_NPRTN = 1;
 constexpr uint32_t LVL3_CHUNKS_PER_WG = 64;
 constexpr uint32_t HOT_EXCLUDE = 64;  // hot rows kept out of the row windows
 
 static int value_size(int vdt) { return vdt == BLCO_F64 ? 8 : 4; }
+
+// Default tiles of 64 nonzeros per chunk of the row-ordered kernel.  A
+// tensor of up to 2^22 nonzeros takes shorter chunks, so that the grid keeps
+// 4096-8192 waves (2 tiles up to 2^20 nonzeros, 4 up to 2^21, 8 up to 2^22):
+// with 16 tiles such a tensor fills a fraction of the 256 CUs, and 2 tiles
+// ran tensors of 1e5-3e5 nonzeros in a third of the time (the total over
+// chunk targets from 2000 to 64000 differs by 2%).  From 2^22 nonzeros 16,
+// and 32 for FP64 rows of 256 bytes or more from 2^26 nonzeros: fewer
+// chunk ends, each of which adds its open rows to HBM (3-7% faster on the
+// 77M-nonzero modes, 0.5% on the 1.7B-nonzero ones; FP32 rows of that size
+// were faster at 16).
+static int sorted_tpw(const blco_dev* d, int rank, int ftype)
+{
+    if (ftype == BLCO_F64 && rank >= 32 && d->nnz >= (1ull << 26)) return 32;
+    int t = 2;
+    while (t < SORTED_TPW && d->nnz > (uint64_t)WAVE * t * SORTED_TARGET_CHUNKS) t *= 2;
+    return t;
+}
 
 const ChunkTab* chunk_table(blco_dev* d, int tpw)
 {
@@ -902,7 +921,7 @@ int mttkrp(blco_dev* d, int mode, int rank, int ftype, void* const* factors, voi
             break;
         default: return fail(BLCO_EINVAL, "unknown kernel id %d", kernel);
     }
-    if (tpw <= 0) tpw = which == W_SORTED ? SORTED_TPW : DEFAULT_TPW;
+    if (tpw <= 0) tpw = which == W_SORTED ? sorted_tpw(d, rank, ftype) : DEFAULT_TPW;
     if (used_kernel) *used_kernel = kernel;
     if (d->nnz == 0) return BLCO_OK;
     if (which == W_SORTED) {

@@ -49,14 +49,16 @@ constexpr int CAND_NPRTN[BLCO_SEL_NCAND] = {1, 1, 1, 1, 2, 4, 8, 16, 32, 64, 1, 
 constexpr int CAND_TPW[BLCO_SEL_NCAND] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 16, 8, 32};
 
 // The network's choice replaces the rule's only when it predicts at least
-// 2^SEL_MARGIN times less time (7%: the selector's held-out regret against
-// the exhaustive choice is 5.6% with it, 5.3% without, and the choices it
-// keeps from the rule are those within the network's error), and only for
-// modes whose size features (log2 of I_n, nnz, blocks, nnz per block, R,
-// sum of the other lengths, largest slice, nnz / I_n) lie within SEL_SIZE_Z
-// standard deviations of the training set's mean (the training data reach
-// 3.6).
+// 2^SEL_MARGIN times less time (7%; the lowest training-set regret among
+// margins 0 to 0.6), only for tensors of at most 10^7 nonzeros (log2 of nnz
+// up to SEL_MAX_LOG2_NNZ: the range of 400 of the 430 training tensors;
+// above it, with 30, the network's log2-time error is 0.6, far beyond the
+// rule's 2.6% regret there), and only for modes whose size features (log2
+// of I_n, nnz, blocks, nnz per block, R, sum of the other lengths, largest
+// slice, nnz / I_n) lie within SEL_SIZE_Z standard deviations of the
+// training set's mean (the training data reach 3.6).
 constexpr float SEL_MARGIN = 0.1f;
+constexpr float SEL_MAX_LOG2_NNZ = 23.25f;
 constexpr float SEL_SIZE_Z = 4.0f;
 constexpr int SIZE_FEATS[] = {0, 1, 2, 3, 4, 7, 16, 22};
 
@@ -228,11 +230,11 @@ bool select_kernel(blco_dev* d, int mode, int rank, int ftype, int* kernel, int*
         }
         float x[BLCO_SEL_NFEAT], y[BLCO_SEL_NCAND];
         mode_features(d, mode, rank, ftype, x);
-        for (int j = 0; j < BLCO_SEL_NFEAT; ++j) x[j] = (x[j] - s.xmean[j]) / s.xstd[j];
         // a mode whose sizes lie outside what the network was trained on
-        // (a size feature beyond SEL_SIZE_Z training standard deviations)
-        // is left to the rule
-        bool inside = true;
+        // (more than 10^7 nonzeros, or a size feature beyond SEL_SIZE_Z
+        // training standard deviations) is left to the rule
+        bool inside = x[1] <= SEL_MAX_LOG2_NNZ;
+        for (int j = 0; j < BLCO_SEL_NFEAT; ++j) x[j] = (x[j] - s.xmean[j]) / s.xstd[j];
         for (int f : SIZE_FEATS) inside = inside && std::fabs(x[f]) <= SEL_SIZE_Z;
         if (inside && s.net && blco_mlp_forward(s.net, x, 1, y) == BLCO_OK) {
             int best = -1;

@@ -18,6 +18,12 @@ recorded.
 
 Writes the file after every tensor, so an interrupted run keeps what it
 measured.  Prints one line per tensor.
+
+After a change to a kernel, --retime re-times the listed candidates of every
+tensor in existing files in place (the tensors are drawn again from the
+specs the files record), leaving the other columns as measured:
+
+  python collect.py --retime 10 --update data/part_000.npz data/part_001.npz
 """
 import argparse
 import importlib.util
@@ -81,6 +87,55 @@ def time_candidate(B, dev, mode, rank, F, out, c):
     return ms
 
 
+def make_tensor(B, sp):
+    """Tensor of spec sp on the device with FP32 values: the GPU generator,
+    or the host generator for keys wider than it takes."""
+    try:
+        return B.DeviceTensor.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                                       seed=sp["seed"], topup=False, value_dtype=1)
+    except B.BlcoError:
+        coo = B.Coo.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
+                             seed=sp["seed"], topup=False)
+        return B.DeviceTensor(B.encode(coo), value_dtype=1)
+
+
+def retime(B, paths, cands, verbose):
+    """Re-times candidates `cands` of every (tensor, mode) row of the data
+    files `paths` in place."""
+    for path in paths:
+        with np.load(path) as z:
+            D = {k: z[k] for k in z.files}
+        meta = json.loads(str(D["meta"]))
+        times = D["times"].copy()
+        for sp in meta:
+            t0 = time.perf_counter()
+            dev = make_tensor(B, sp)
+            if int(dev.nnz) != sp["nnz_unique"]:
+                raise RuntimeError("tensor %d: %d nonzeros, the file records %d"
+                                   % (sp["t"], dev.nnz, sp["nnz_unique"]))
+            R = sp["rank"]
+            F = [ones(B, d, R) for d in sp["dims"]]
+            rows = np.nonzero(D["tensor"] == sp["t"])[0]
+            for i in rows:
+                m = int(D["mode"][i])
+                out = B.DeviceArray((sp["dims"][m], R), B.F32)
+                for c in cands:
+                    old = times[i, c]
+                    times[i, c] = time_candidate(B, dev, m, R, F, out, c) if dev.candidate_valid(m, R, B.F32, c) \
+                        else np.nan
+                    if verbose:
+                        print("  tensor %d mode %d candidate %d: %.3f -> %.3f ms" % (sp["t"], m, c, old, times[i, c]),
+                              file=sys.stderr, flush=True)
+                out.free()
+            for f in F:
+                f.free()
+            dev.free()
+            D["times"] = times
+            np.savez(path, **D)
+            print("%s tensor %d: candidates %s re-timed in %.1f s" % (os.path.basename(path), sp["t"], cands,
+                                                                     time.perf_counter() - t0), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--first", type=int, default=0)
@@ -90,26 +145,29 @@ def main():
     ap.add_argument("--nprtn-rule", type=int, default=16,
                     help="lvl3 partitions of the -k 10 rule (the reference's default is the "
                          "OpenMP thread count)")
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out")
     ap.add_argument("--verbose", action="store_true", help="one line per timed candidate (stderr)")
+    ap.add_argument("--retime", default="", help="comma list of candidates to re-time in the --update files")
+    ap.add_argument("--update", nargs="+", default=[], help="data files whose --retime columns are re-timed")
     args = ap.parse_args()
     B = load_pkg()
     if B.device_count() < 1:
         sys.exit("collect.py needs a GPU")
+    if args.update:
+        cands = [int(c) for c in args.retime.split(",") if c]
+        if not cands or any(c < 0 or c >= B.SEL_NCAND for c in cands):
+            sys.exit("--update needs --retime with candidates in [0, %d)" % B.SEL_NCAND)
+        retime(B, args.update, cands, args.verbose)
+        return
+    if not args.out:
+        sys.exit("--out is required")
     rows = dict(feat=[], times=[], tensor=[], mode=[], rank=[], rule=[])
     meta = []
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     for t in range(args.first, args.first + args.count):
         sp = tensor_spec(t, args.nnz_min, args.nnz_max)
         t0 = time.perf_counter()
-        try:
-            dev = B.DeviceTensor.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
-                                          seed=sp["seed"], topup=False, value_dtype=1)
-        except B.BlcoError:  # keys wider than the GPU generator takes: the host generator
-            coo = B.Coo.generate(sp["dims"], sp["nnz"], zipf=sp["zipf"], gen_rank=8, value_kind=0,
-                                 seed=sp["seed"], topup=False)
-            dev = B.DeviceTensor(B.encode(coo), value_dtype=1)
-            del coo
+        dev = make_tensor(B, sp)
         R = sp["rank"]
         F = [ones(B, d, R) for d in sp["dims"]]
         tgen = time.perf_counter() - t0

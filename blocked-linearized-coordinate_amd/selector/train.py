@@ -77,14 +77,15 @@ def choose(pred, valid):
 
 # The library's decision (csrc/hip/selector.hip): the rule's candidate unless
 # the network predicts 2^MARGIN times less time, and the rule outright for
-# modes whose size features lie beyond SIZE_Z training standard deviations.
-MARGIN, SIZE_Z, SIZE_FEATS = 0.1, 4.0, [0, 1, 2, 3, 4, 7, 16, 22]
+# tensors of more than 2^MAX_LOG2_NNZ nonzeros and for modes whose size
+# features lie beyond SIZE_Z training standard deviations.
+MARGIN, MAX_LOG2_NNZ, SIZE_Z, SIZE_FEATS = 0.1, 23.25, 4.0, [0, 1, 2, 3, 4, 7, 16, 22]
 
 
-def choose_lib(pred, valid, rule, Xn):
+def choose_lib(pred, valid, rule, X, Xn):
     best = choose(pred, valid)
     idx = np.arange(len(pred))
-    inside = (np.abs(Xn[:, SIZE_FEATS]) <= SIZE_Z).all(axis=1)
+    inside = (X[:, 1] <= MAX_LOG2_NNZ) & (np.abs(Xn[:, SIZE_FEATS]) <= SIZE_Z).all(axis=1)
     rule_ok = valid[idx, rule]
     gain = np.where(rule_ok, pred[idx, rule] - pred[idx, best], np.inf)
     return np.where(inside & (gain > MARGIN), best, np.where(rule_ok, rule, best))
@@ -128,7 +129,7 @@ def main():
         if not sel.any():
             continue
         T = D["times"][sel]
-        rep[name] = dict(learned=regret(T, choose_lib(pred[sel], valid[sel], D["rule"][sel], Xn[sel])),
+        rep[name] = dict(learned=regret(T, choose_lib(pred[sel], valid[sel], D["rule"][sel], X[sel], Xn[sel])),
                          learned_argmin=regret(T, choose(pred[sel], valid[sel])),
                          rule_k10=regret(T, D["rule"][sel]),
                          always_window=regret(T, np.zeros(sel.sum(), int)),

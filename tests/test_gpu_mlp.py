@@ -101,17 +101,20 @@ def test_selector_install_and_choice(B, gpu):
     assert np.isclose(f[0], np.log2(3000)) and f[4] == 4 and f[5] == 3 and f[6] == 2
     assert abs(f[8:16].sum() - 1) < 1e-5 and abs(f[18:22].sum() - 1) < 1e-5
     assert f[23] == 0 and dev.features(0, 16, B.F32)[23] == 1
-    # a network with zero weights predicts its last-layer biases
+    # a network with zero weights predicts its last-layer biases; a feature
+    # scale of 100 keeps every feature within the selector's size guard
     widths = [B.SEL_NFEAT, 8, B.SEL_NCAND]
     nparams = B.SEL_NFEAT * 8 + 8 + 8 * B.SEL_NCAND + B.SEL_NCAND
-    xm, xs = np.zeros(B.SEL_NFEAT), np.ones(B.SEL_NFEAT)
+    xm, xs = np.zeros(B.SEL_NFEAT), np.full(B.SEL_NFEAT, 100.0)
 
-    def install(pref):
+    def install(pref, scale=xs, bias=None):
         p = np.zeros(nparams, np.float32)
         p[-B.SEL_NCAND:] = 5.0
         for rank, c in enumerate(pref):
             p[nparams - B.SEL_NCAND + c] = rank
-        B.selector_install(widths, p, xm, xs)
+        for c, v in (bias or {}).items():
+            p[nparams - B.SEL_NCAND + c] = v
+        B.selector_install(widths, p, xm, scale)
 
     try:
         install([6, 2])  # lvl3 with 8 partitions, then atomic
@@ -121,6 +124,16 @@ def test_selector_install_and_choice(B, gpu):
         assert dev.select_kernel(0, 16, B.F32) == (B.K_BATCHED, 1, 0)
         install([12])  # segmented lvl1 with 16 tiles per wave
         assert dev.select_kernel(0, 16, B.F32) == (B.K_BATCHED, 1, 16)
+        # the rule's candidate (mode 0: lvl3 with 16 partitions) is kept
+        # unless another is predicted 2^0.1 times faster
+        install([], bias={6: 1.0, 7: 1.05})
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_LVL3, 16, 0)
+        install([], bias={6: 1.0, 7: 1.3})
+        assert dev.select_kernel(0, 16, B.F32) == (B.K_LVL3, 8, 0)
+        # features beyond 4 training standard deviations (unit scale: log2
+        # of 50000 nonzeros is 15.6) leave the mode to the rule
+        install([6, 2], scale=np.ones(B.SEL_NFEAT))
+        assert dev.select_kernel(0, 16, B.F32) == (dev.auto_kernel(0, 16), 0, 0)
         B.selector_install(None, None, None, None)
         assert dev.select_kernel(1, 16, B.F32) == (dev.auto_kernel(1, 16), 0, 0)
     finally:
@@ -146,12 +159,18 @@ def test_timed_mttkrp_reports_the_kernel(B, gpu):
 def test_selector_training_from_committed_timings(B, gpu):
     """Trains the selector network with the library's kernels on the
     committed timings of the training tensors (0..399 and 500..529) and
-    checks the choice on the held-out ones (400..499 and 530..539):
-    total-time regret against the exhaustive choice <= 10%, and below the
-    -k 10 rule's."""
+    checks the library's decision (the rule's candidate unless the network
+    predicts a clear gain, within the trained sizes) on the held-out ones
+    (400..499 and 530..539): total-time regret against the exhaustive choice
+    <= 10%, and below the -k 10 rule's."""
     import glob
-    sel = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                       "blocked-linearized-coordinate_amd", "selector", "data")
+    import importlib.util
+    seldir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "blocked-linearized-coordinate_amd", "selector")
+    spec = importlib.util.spec_from_file_location("blco_selector_train", os.path.join(seldir, "train.py"))
+    Tr = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(Tr)
+    sel = os.path.join(seldir, "data")
     parts = [np.load(f) for f in sorted(glob.glob(os.path.join(sel, "*.npz")))]
     D = {k: np.concatenate([p[k] for p in parts]) for k in ("feat", "times", "tensor", "rule")}
     valid = ~np.isnan(D["times"])
@@ -169,7 +188,7 @@ def test_selector_training_from_committed_timings(B, gpu):
     pred = net.forward(Xn[te])
     net.free()
     T = D["times"][te]
-    choice = np.argmin(np.where(valid[te], pred, np.inf), axis=1)
+    choice = Tr.choose_lib(pred, valid[te], D["rule"][te], X[te], Xn[te])
     best = np.nanmin(T, axis=1)
     learned = T[np.arange(len(T)), choice].sum() / best.sum() - 1
     rule = T[np.arange(len(T)), D["rule"][te]].sum() / best.sum() - 1

@@ -406,8 +406,17 @@ def test_cpd_pivot_floor_is_order_independent(B, O):
     a1 = O.Alto(dims, t.cidx, t.vals, nprtn=1)
     a8 = O.Alto(dims, t.cidx, t.vals, nprtn=8)
     r1 = a1.cpd(U0, 12, 0.0)[2]
-    r8 = a8.cpd(U0, 12, 0.0)[2]
-    assert np.abs(r1 - r8).max() > 1e-3
+    # the 8-partition MTTKRP adds with atomics on the short-fiber modes (as
+    # the reference's mttkrp_alto_atomic does), so its summation order varies
+    # between runs and now and then (about 1 run in 8 here) equals the
+    # 1-partition order: one of up to five runs diverges
+    diverged = False
+    for _ in range(5):
+        r8 = a8.cpd(U0, 12, 0.0)[2]
+        if np.abs(r1 - r8).max() > 1e-3:
+            diverged = True
+            break
+    assert diverged
     for pe in (2.0 ** -52, 2.0 ** -23):
         f1 = a1.cpd(U0, 12, 0.0, pivot_eps=pe)[2]
         f8 = a8.cpd(U0, 12, 0.0, pivot_eps=pe)[2]
