@@ -333,7 +333,10 @@ def run_large(B, args):
         "first_iter_s_fp32": round(m["first_iter_s"], 2),
         "device_gb_fp32": round(m["device_gb"], 2),
         "fit_fp32": float(m["fits"][-1]),
-        "roofline": roofline_json(m, m["kernels"].tolist(), "large-1.7B", 4, 4),
+        # the committed traffic profile covers the full-size tensor only
+        "roofline": roofline_json(m, m["kernels"].tolist(),
+                                  "large-1.7B" if args.large_scale == 1.0 else "large-1.7B-scale%g" % args.large_scale,
+                                  4, 4),
     }
     if "fp64" in res:
         r64 = res["fp64"]
