@@ -111,8 +111,9 @@ def retime(B, paths, cands, verbose):
             t0 = time.perf_counter()
             dev = make_tensor(B, sp)
             if int(dev.nnz) != sp["nnz_unique"]:
-                raise RuntimeError("tensor %d: %d nonzeros, the file records %d"
-                                   % (sp["t"], dev.nnz, sp["nnz_unique"]))
+                got = int(dev.nnz)
+                dev.free()
+                raise RuntimeError("tensor %d: %d nonzeros, the file records %d" % (sp["t"], got, sp["nnz_unique"]))
             R = sp["rank"]
             F = [ones(B, d, R) for d in sp["dims"]]
             rows = np.nonzero(D["tensor"] == sp["t"])[0]

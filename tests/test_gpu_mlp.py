@@ -193,3 +193,41 @@ def test_selector_training_from_committed_timings(B, gpu):
     learned = T[np.arange(len(T)), choice].sum() / best.sum() - 1
     rule = T[np.arange(len(T)), D["rule"][te]].sum() / best.sum() - 1
     assert learned <= 0.10 and learned < rule, (learned, rule)
+
+
+def test_collect_retime_updates_only_the_listed_candidates(B, gpu, tmp_path):
+    """collect.py --retime: the tensors recorded in a timing file are drawn
+    again from their specs and only the listed candidates' columns are
+    re-timed; a spec whose tensor no longer matches the recorded nonzero
+    count is refused."""
+    import importlib.util
+    import json
+    seldir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "blocked-linearized-coordinate_amd", "selector")
+    spec = importlib.util.spec_from_file_location("blco_selector_collect", os.path.join(seldir, "collect.py"))
+    Col = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(Col)
+    sp = dict(t=7, seed=123457, dims=[50, 60, 70], nnz=5000, zipf=[0.0, 0.5, 1.0], rank=8)
+    dev = Col.make_tensor(B, sp)
+    sp.update(nnz_unique=int(dev.nnz), blocks=int(dev.nblocks))
+    dev.free()
+    times = np.full((3, B.SEL_NCAND), 1000.0)
+    path = str(tmp_path / "part.npz")
+
+    def write(meta):
+        np.savez(path, feat=np.zeros((3, B.SEL_NFEAT), np.float32), times=times, tensor=np.full(3, 7),
+                 mode=np.arange(3), rank=np.full(3, 8), rule=np.full(3, 10), meta=np.array(json.dumps(meta)))
+
+    write([sp])
+    Col.retime(B, [path], [10, 13], False)
+    with np.load(path) as z:
+        T = z["times"]
+        assert json.loads(str(z["meta"])) == [sp] and list(z["mode"]) == [0, 1, 2]
+    for c in range(B.SEL_NCAND):
+        if c in (10, 13):
+            assert np.all((T[:, c] > 0) & (T[:, c] < 1000.0)), (c, T[:, c])
+        else:
+            assert np.all(T[:, c] == 1000.0), c
+    write([dict(sp, nnz_unique=sp["nnz_unique"] + 1)])
+    with pytest.raises(RuntimeError):
+        Col.retime(B, [path], [10], False)
