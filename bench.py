@@ -143,6 +143,26 @@ def copy_calibration(B, nbytes):
     dst.free()
 
 
+def copy_ceiling(B, nbytes=1 << 32, reps=10):
+    """Practical HBM ceiling beside the published 8 TB/s: a device-to-device
+    copy of nbytes (the runtime's copy kernel), bytes read plus bytes
+    written per second, host-timed over reps copies after one untimed."""
+    src = B.DeviceArray((nbytes // 8,), B.F64)
+    dst = B.DeviceArray((nbytes // 8,), B.F64)
+    B._check(B.lib().blco_memcpy_dtod(dst.ptr, src.ptr, src.nbytes))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        B._check(B.lib().blco_memcpy_dtod(dst.ptr, src.ptr, src.nbytes))
+    t = (time.perf_counter() - t0) / reps
+    src.free()
+    dst.free()
+    gbs = 2 * nbytes / t / 1e9
+    log("HBM copy ceiling: %.0f GB/s (%d-byte device copy, read + write)" % (gbs, nbytes))
+    return {"gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "what": "device-to-device copy of %d bytes (runtime copy kernel), bytes read + written per second, "
+                    "host-timed mean of %d" % (nbytes, reps)}
+
+
 def mttkrp_outputs(B, dev, U0, rank, ftype):
     """MTTKRP of every mode from the factors U0 (host f64, converted to
     ftype), copied back as f64."""
@@ -251,6 +271,8 @@ def run_large(B, args):
     U0 = B.init_factors(W["seed"], dims, R)
     if args.copy_calibration:
         copy_calibration(B, args.copy_calibration)
+    # (left out of counter runs, whose scaling reads the calibration copy)
+    ceiling = None if args.copy_calibration else copy_ceiling(B)
     forms = [f for f in (args.large_forms or "fp64,fp32").split(",") if f]
     res = {}
     out64 = None
@@ -337,6 +359,7 @@ def run_large(B, args):
         "roofline": roofline_json(m, m["kernels"].tolist(),
                                   "large-1.7B" if args.large_scale == 1.0 else "large-1.7B-scale%g" % args.large_scale,
                                   4, 4),
+        "hbm_copy_ceiling": ceiling,
     }
     if "fp64" in res:
         r64 = res["fp64"]
@@ -544,6 +567,8 @@ def run_workload(B, args):
     nnz = coo.nnz
     if args.copy_calibration:
         copy_calibration(B, args.copy_calibration)
+    # (left out of counter runs, whose scaling reads the calibration copy)
+    ceiling = None if args.copy_calibration else copy_ceiling(B)
     results = {}
     forms = [("fp32", B.F32, 1)] + ([] if args.no_fp64 else [("fp64", B.F64, 0)])
     errs = None
@@ -589,6 +614,7 @@ def run_workload(B, args):
         "mttkrp_ms_per_mode_fp32": [round(float(x), 4) for x in m["per_mode_ms"]],
         "fit_fp32": float(m["fits"][-1]),
         "roofline": roofline_json(m, m["kernels"].tolist(), args.workload, 4, 4),
+        "hbm_copy_ceiling": ceiling,
     }
     if errs is not None:
         out["mttkrp_rel_err_fp32_vs_fp64_oracle_per_mode"] = [float("%.3e" % e) for e in errs]
