@@ -49,11 +49,12 @@ constexpr int CAND_NPRTN[BLCO_SEL_NCAND] = {1, 1, 1, 1, 2, 4, 8, 16, 32, 64, 1, 
 constexpr int CAND_TPW[BLCO_SEL_NCAND] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 16, 8, 32};
 
 // The network's choice replaces the rule's only when it predicts at least
-// 2^SEL_MARGIN times less time (7%; the lowest training-set regret among
-// margins 0 to 0.6), only for tensors of at most 10^7 nonzeros (log2 of nnz
+// 2^SEL_MARGIN times less time (7%; the training-set regrets of margins 0
+// to 0.6 lie within 0.9 points of each other, 4.1-5.0%, and 0.1 is within
+// 0.2 of the lowest), only for tensors of at most 10^7 nonzeros (log2 of nnz
 // up to SEL_MAX_LOG2_NNZ: the range of 400 of the 430 training tensors;
-// above it, with 30, the network's log2-time error is 0.6, far beyond the
-// rule's 2.6% regret there), and only for modes whose size features (log2
+// above it, with 30, the network's held-out log2-time error is 0.54, far
+// beyond the rule's 3.6% regret there), and only for modes whose size features (log2
 // of I_n, nnz, blocks, nnz per block, R, sum of the other lengths, largest
 // slice, nnz / I_n) lie within SEL_SIZE_Z standard deviations of the
 // training set's mean (the training data reach 3.6).
