@@ -161,7 +161,9 @@ int blco_dev_set_value_dtype(blco_dev* d, int value_dtype);
  * (BLCO_F64, BLCO_F32 or BLCO_BF16); out: device dims[mode] x rank, f64 for
  * BLCO_F64 factors, f32 otherwise; it is accumulated into (the caller
  * zeroes it, src/cpd_gpu.cu:155).  nprtn: lvl3 partial-matrix count (0 =
- * default); thread_cf: tiles per wave (0 = default). */
+ * default); thread_cf: tiles of 64 nonzeros per wave (0 = default: 4, and
+ * for kernel 14 16, or 2/4/8 for tensors of up to 2^20/2^21/2^22
+ * nonzeros, or 32 for FP64 factors at rank >= 32 from 2^26 nonzeros). */
 int blco_mttkrp(blco_dev* d, int mode, int rank, int factor_dtype,
                 void* const* factors, void* out, int kernel, int nprtn,
                 int thread_cf, void* stream);
