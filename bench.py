@@ -451,6 +451,8 @@ def run_batched(B, args):
         log("tensor %d %s: batch fit %r (%d iterations), plain fit %r"
             % (k, tens[k][0].dims, fits[k], its[k], plain_fits[k]))
     dfit = float(np.max(np.abs(fits - plain_fits)))
+    order = np.argsort(-np.abs(fits - plain_fits))
+    top = [[int(k), float("%.3e" % abs(fits[k] - plain_fits[k]))] for k in order[:3]]
     log("batched: %.3f ms per sweep of all tensors (plain per-tensor sum %.3f ms), max fit diff %.2e"
         % (ms, plain_ms, dfit))
     out = {
@@ -464,6 +466,7 @@ def run_batched(B, args):
         "plain_s_per_iter": round(plain_ms / 1e3, 6),
         "speedup_vs_plain": round(plain_ms / ms, 3) if ms > 0 else None,
         "max_fit_diff_vs_plain": dfit,
+        "largest_fit_diffs": top,  # [tensor index, |batch fit - plain fit|]
         "host_wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
