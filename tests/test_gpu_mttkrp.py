@@ -96,6 +96,29 @@ def test_row_ordered_all_shapes(B, O, gpu, tensors, name):
     check(B, O, tensors, name, 0, B.K_SORTED, 3, tpw=1)
 
 
+@pytest.mark.parametrize("nnz", [1_500_000, 3_000_000])
+def test_row_ordered_default_chunk_lengths(B, O, gpu, nnz):
+    """Kernel 14 at its default chunk length for tensors between 2^20 and
+    2^22 nonzeros (4 and 8 tiles of 64 per wave; the small shapes above take
+    2 and the 40M-nonzero tensor 16), FP64 at rank 32 and FP32 at rank 20,
+    on a 4-mode tensor with Zipf-skewed modes."""
+    dims = (3000, 2000, 500, 40)
+    t = B.Coo.generate(dims, nnz, zipf=(1.1, 0.8, 0.0, 0.0), gen_rank=5, value_kind=1, seed=nnz % 9973,
+                       topup=True)
+    assert t.nnz == nnz
+    at = O.Alto(dims, t.cidx, t.vals, nprtn=8)
+    h = B.encode(t)
+    for dtype, rank in ((0, 32), (1, 20)):
+        dev = B.DeviceTensor(h, value_dtype=dtype)
+        rng = np.random.default_rng(rank)
+        U = [rng.random((d, rank)) for d in dims]
+        print("default chunk length: nnz", nnz, "dtype", dtype, "rank", rank, flush=True)
+        for m in range(len(dims)):
+            e = rel(run_mttkrp(B, dev, U, m, dtype, B.K_SORTED), at.mttkrp(U, m))
+            assert e <= TOL[dtype], (nnz, dtype, rank, m, e)
+        dev.free()
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_row_ordered_xcd_tiles(B, O, gpu, tensors, name, monkeypatch):
     """Kernel 14 over copies grouped by 8 tiles of the split mode (forced on
